@@ -1,0 +1,170 @@
+"""Throughput of the distance-field path on one MI355X.
+
+Workload (the production shape): bumpy icosphere split 8 times (1,310,720 triangles, 32 bumps),
+one step = a 256^3 signed-distance volume (calcFromGridWithDistances, FWN beta 2) + findProjections
+of 2^22 query points (half uniform in the bbox grown 20 %, half in a Gaussian shell of sigma =
+2 % of the bbox diagonal).  Inputs are resident in HBM before the timed region.
+
+Prints ONE JSON line: value = (voxels + projected points) per second for the whole step, with the
+per-part rates, the roofline of the dominant kernel (signed-distance grid, HBM-bound pointer
+chasing; algorithmic bytes per voxel from the reference-order traversal counts of the CPU oracle
+on a seeded sample) and the CPU baseline (the oracle, a C restatement of the reference CPU path,
+timed on the host cores on a bounded sample of the same workload).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+MESH_LEVEL, MESH_BUMPS, MESH_SEED = 8, 32, 2
+GRID = 256
+NQ = 1 << 22
+QSEED = 7
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only: this build has no multi-GPU path (asked for --gpus "
+              f"{args.gpus}); the path shards by z-slabs / query ranges with no exchange, see DESIGN.md",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import meshlib_amd as M
+    from meshlib_amd import synth
+
+    torch.cuda.set_device(0)
+    t0 = time.time()
+    v, f = synth.bumpy_icosphere(MESH_LEVEL, MESH_BUMPS, MESH_SEED)
+    dims, xf, vox = synth.grid_for_mesh(v, (GRID, GRID, GRID))
+    pts = synth.projection_queries(v, f, NQ, QSEED)
+    log(f"inputs {time.time() - t0:.1f}s: {f.shape[0]} tris, {GRID}^3 voxels, {NQ} queries")
+    t0 = time.time()
+    mesh = M.Mesh(v, f)
+    log(f"tree build + upload {time.time() - t0:.1f}s (depth {mesh.depth()})")
+    fwn = M.FastWindingNumber(mesh)
+    proj = M.PointsToMeshProjector()
+    proj.updateMeshData(mesh)
+    opts = M.DistanceToMeshOptions()
+    nvox = GRID ** 3
+    d_pts = torch.from_numpy(pts).cuda()
+    d_sdf = torch.empty(nvox, dtype=torch.float32, device="cuda")
+    d_proj = torch.empty(NQ * 8, dtype=torch.int32, device="cuda")
+    stream = torch.cuda.current_stream()
+    sh = stream.cuda_stream
+
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    sdf_ms, proj_ms = [], []
+
+    def step(timed):
+        ev[0].record(stream)
+        fwn.sdf_grid_device(dims, xf, opts, d_sdf.data_ptr(), stream=sh)
+        ev[1].record(stream)
+        proj.find_projections_device(d_pts.data_ptr(), NQ, d_proj.data_ptr(), stream=sh)
+        ev[2].record(stream)
+        if timed:
+            return True
+
+    for i in range(args.warmup):
+        step(False)
+        torch.cuda.synchronize()
+        log(f"warmup step {i} done")
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        step(True)
+        torch.cuda.synchronize()
+        sdf_ms.append(ev[0].elapsed_time(ev[1]))
+        proj_ms.append(ev[1].elapsed_time(ev[2]))
+        log(f"step {i}: sdf {sdf_ms[-1]:.2f} ms, projections {proj_ms[-1]:.2f} ms")
+    torch.cuda.synchronize()
+    t_total = time.perf_counter() - t_start
+    ms_per_step = 1000.0 * t_total / args.steps
+    sdf_avg = float(np.mean(sdf_ms))
+    proj_avg = float(np.mean(proj_ms))
+    vox_rate = nvox / (sdf_avg / 1000.0)
+    pts_rate = NQ / (proj_avg / 1000.0)
+    value = (nvox + NQ) / (ms_per_step / 1000.0)
+
+    # ---- CPU baseline (the oracle: C restatement of the reference CPU path) + algorithmic bytes
+    cpu = None
+    b_vox = b_pt = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        cores = max(1, min(16, os.cpu_count() or 1))
+        om = O.Mesh(v, f)
+        rng = np.random.default_rng(123)
+        budget = args.cpu_seconds / 2
+        cv, cp = O.Counters(), O.Counters()
+        n_v = n_p = 0
+        t_v = t_p = 0.0
+        while t_v < budget:
+            ids = rng.choice(nvox, 2000 * cores, replace=False).astype(np.int64)
+            t = time.perf_counter()
+            om.sdf_grid_sample(dims, xf, ids, nthreads=cores, counters=cv)
+            t_v += time.perf_counter() - t
+            n_v += ids.size
+        while t_p < budget:
+            sel = rng.choice(NQ, 4000 * cores, replace=False)
+            t = time.perf_counter()
+            om.find_projections(pts[sel], nthreads=cores, counters=cp)
+            t_p += time.perf_counter() - t
+            n_p += sel.size
+        cpu_vox_rate = n_v / t_v
+        cpu_pts_rate = n_p / t_p
+        cpu_step = nvox / cpu_vox_rate + NQ / cpu_pts_rate
+        cv_d, cp_d = cv.as_dict(), cp.as_dict()
+        # algorithmic bytes per unit (reference AoS record sizes): query 12 + result 32, 32 B per
+        # child-box probe, 48 B per leaf test; voxel: 4 B out, the distance part, and 64 B per FWN
+        # node visit + 48 B per solid-angle leaf
+        b_pt = 12 + 32 + (32 * cp_d["dist_probes"] + 48 * cp_d["dist_leaves"]) / cp_d["queries"]
+        b_vox = 4 + (32 * cv_d["dist_probes"] + 48 * cv_d["dist_leaves"] + 64 * cv_d["fwn_visits"]
+                     + 48 * cv_d["fwn_leaves"]) / cv_d["queries"]
+        cpu = {"value": (nvox + NQ) / cpu_step, "unit": "queries/s", "cores": cores, "kind": "port",
+               "sample": f"{n_v} seeded voxels of the 256^3 grid + {n_p} seeded queries of the 2^22 set, "
+                         f"{t_v + t_p:.1f}s of CPU time; step time extrapolated as voxels/rate + points/rate",
+               "voxels_per_s": cpu_vox_rate, "pts_per_s": cpu_pts_rate}
+        log(f"cpu baseline: {cpu_vox_rate:.0f} vox/s, {cpu_pts_rate:.0f} pts/s on {cores} threads")
+
+    roof = None
+    if b_vox is not None:
+        achieved = b_vox * nvox / (sdf_avg / 1000.0) / 1e9
+        roof = {"bound": "hbm", "kernel": "kSdfGrid", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "bytes_per_voxel": b_vox, "bytes_per_point": b_pt}
+    out = {
+        "metric": "signed-dist voxels/s & projected pts/s; learned-bound vs plain GPU speedup",
+        "value": value, "unit": "queries/s (voxels + projected points)",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": "bumpy icosphere k=8 (1,310,720 tris, 32 bumps): 256^3 signed-distance "
+                               "grid + 2^22 point projections per step", "voxels": nvox,
+                   "points": NQ, "beta": 2.0},
+        "voxels_per_s": vox_rate, "pts_per_s": pts_rate, "sdf_ms": sdf_avg, "proj_ms": proj_avg,
+        "learned_vs_plain": None,
+        "roofline": roof, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,164 @@
+/*
+ * meshlib_amd.h -- C-ABI of the MI355X (gfx950) distance-field path.
+ *
+ * Plain pointers and sizes, no C++ or torch types.  Every entry point returns
+ * MI_OK (0) on success or a negative MI_E_* status; mi_last_error() returns the
+ * calling thread's last message.  Host-pointer entry points behave like the
+ * reference's synchronous calls (upload, compute, download; chunked to fit
+ * device memory).  *_device entry points take device pointers that are already
+ * resident in HBM and an optional hipStream_t (NULL = default stream); they are
+ * asynchronous with respect to the host.
+ *
+ * Each declaration names the reference interface it replaces.
+ */
+#ifndef MESHLIB_AMD_H
+#define MESHLIB_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MI_OK 0
+#define MI_E_INVALID (-1)   /* bad argument */
+#define MI_E_HIP (-2)       /* HIP runtime error ("AMD GPU error: ...") */
+#define MI_E_CANCELED (-3)  /* progress callback returned 0 (operation canceled) */
+#define MI_E_NOMEM (-4)     /* host allocation failed */
+#define MI_E_LIMIT (-5)     /* input exceeds a fixed capacity of the entry point */
+
+/* DistanceToMeshOptions (source/MRMesh/MRDistanceToMeshOptions.h:10-33) */
+typedef struct mi_dist_options {
+    float minDistSq;               /* default 0 */
+    float maxDistSq;               /* default FLT_MAX */
+    int32_t nullOutsideMinMax;     /* default 1 */
+    float windingNumberThreshold;  /* default 0.5 */
+    float windingNumberBeta;       /* default 2 */
+} mi_dist_options;
+
+/* MeshProjectionResult wire record, 32 B (source/MRMesh/MRMeshProject.h:17-31,
+ * GPU twin source/MRCuda/MRCudaPointsToMeshProjector.cuh:12-17).  face = -1 when
+ * nothing was found below upDistLimitSq (then distSq = upDistLimitSq).  e is always
+ * -1 here: edgeWithLeft(face) is topology, filled by the host adapter
+ * (source/MRCuda/MRCudaPointsToMeshProjector.cpp:118-124). */
+typedef struct mi_projection {
+    int32_t face;
+    float point[3];
+    int32_t e;
+    float a, b;
+    float distSq;
+} mi_projection;
+
+/* AABB tree node in the reference's pre-order layout, 32 B
+ * (source/MRMesh/MRAABBTreeNode.h:29-45): leaf <=> r < 0, then l = face id. */
+typedef struct mi_tree_node {
+    float min[3], max[3];
+    int32_t l, r;
+} mi_tree_node;
+
+/* Dipole, 32 B (source/MRMesh/MRDipole.h:10-30) */
+typedef struct mi_dipole {
+    float pos[3];
+    float area;
+    float dirArea[3];
+    float rr;
+} mi_dipole;
+
+/* ProgressCallback: return 0 to cancel */
+typedef int (*mi_progress_fn)(float progress, void* user);
+
+/* IFastWindingNumberByParts::GridByPartsFunc (source/MRMesh/MRFastWindingNumber.h:88):
+ * receives one block of z-layers, dims = {dims.x, dims.y, layers}, starting at layer
+ * zOffset; the data pointer is valid only during the call.  Return MI_OK to continue. */
+typedef int (*mi_grid_part_fn)(const float* data, const int32_t dims[3], int32_t zOffset, void* user);
+
+typedef struct mi_mesh mi_mesh;
+
+/* ---------------------------------------------------------------- errors / config */
+
+/* copies the calling thread's last error message into buf; returns its full length */
+size_t mi_last_error(char* buf, size_t len);
+
+/* getCudaSafeMemoryLimit (source/MRCuda/MRCudaBasic.cpp:86-92) replacement:
+ * 0 = use 0.8 * (free - 128 MiB); otherwise cap the chunk budget of the host-pointer
+ * entry points at this many bytes (used by tests to force multi-chunk execution) */
+int mi_set_memory_limit(size_t bytes);
+
+/* ---------------------------------------------------------------- mesh data */
+
+/* Cuda::FastWindingNumber::prepareData_ (source/MRCuda/MRCudaFastWindingNumber.cpp:46-85)
+ * and Cuda::PointsToMeshProjector::updateMeshData (source/MRCuda/MRCudaPointsToMeshProjector.cpp:31-50):
+ * verts float[nv*3], tris int32[nf*3] (getTriangulation order).  Builds the AABB tree
+ * (source/MRMesh/MRAABBTree.cpp:31-60) and dipoles (source/MRMesh/MRDipole.cpp:23-71) on the
+ * host and uploads the flattened tree to the current HIP device. */
+int  mi_mesh_create(const float* verts, int32_t nv, const int32_t* tris, int32_t nf, mi_mesh** out);
+void mi_mesh_destroy(mi_mesh* m);
+int  mi_mesh_num_nodes(const mi_mesh* m);
+/* exports the host tree / dipoles in the reference layout (AABBTree::nodes(), Mesh::getDipoles()) */
+int  mi_mesh_get_tree(const mi_mesh* m, mi_tree_node* nodes, mi_dipole* dipoles);
+/* depth of the tree (longest root-to-leaf path, in edges) */
+int  mi_mesh_depth(const mi_mesh* m);
+/* the same host tree + dipole build without any device (AABBTree ctor + calcDipoles):
+ * fills nodes[2*nf-1] and dipoles[2*nf-1] (either may be NULL) and *depth */
+int  mi_build_tree_host(const float* verts, int32_t nv, const int32_t* tris, int32_t nf,
+                        mi_tree_node* nodes, mi_dipole* dipoles, int32_t* depth);
+
+/* ---------------------------------------------------------------- IFastWindingNumber */
+
+/* IFastWindingNumber::calcFromVector (source/MRMesh/MRFastWindingNumber.h:26;
+ * GPU source/MRCuda/MRCudaFastWindingNumber.cpp:87-127); skipFace = -1 for none */
+int mi_fwn_calc_from_vector(mi_mesh* m, const float* pts, int64_t n, float beta, int32_t skipFace,
+                            float* res, mi_progress_fn cb, void* user);
+/* IFastWindingNumber::calcSelfIntersections (MRFastWindingNumber.h:31;
+ * GPU MRCudaFastWindingNumber.cpp:129-173): res[f] = 1 if wn(centroid f, skip f) outside [0,1] */
+int mi_fwn_calc_self_intersections(mi_mesh* m, float beta, uint8_t* res, mi_progress_fn cb, void* user);
+/* IFastWindingNumber::calcFromGrid (MRFastWindingNumber.h:38; GPU MRCudaFastWindingNumber.cpp:175-228);
+ * xf = gridToMeshXf as float[12]: row-major A then b; voxel ids x-fastest */
+int mi_fwn_calc_from_grid(mi_mesh* m, const int32_t dims[3], const float xf[12], float beta,
+                          float* res, mi_progress_fn cb, void* user);
+/* IFastWindingNumber::calcFromGridWithDistances (MRFastWindingNumber.h:44;
+ * GPU MRCudaFastWindingNumber.cpp:230-283) */
+int mi_fwn_calc_from_grid_with_distances(mi_mesh* m, const int32_t dims[3], const float xf[12],
+                                         const mi_dist_options* opts, float* res,
+                                         mi_progress_fn cb, void* user);
+/* IFastWindingNumberByParts::calcFromGridByParts / calcFromGridWithDistancesByParts
+ * (MRFastWindingNumber.h:96-106; GPU MRCudaFastWindingNumber.cpp:285-429) */
+int mi_fwn_calc_from_grid_by_parts(mi_mesh* m, mi_grid_part_fn fn, void* fnUser, const int32_t dims[3],
+                                   const float xf[12], float beta, int32_t layerOverlap,
+                                   mi_progress_fn cb, void* user);
+int mi_fwn_calc_from_grid_with_distances_by_parts(mi_mesh* m, mi_grid_part_fn fn, void* fnUser,
+                                                  const int32_t dims[3], const float xf[12],
+                                                  const mi_dist_options* opts, int32_t layerOverlap,
+                                                  mi_progress_fn cb, void* user);
+
+/* device-resident variants (no chunking, no host copies).  The grid variants compute the
+ * z-layers [zBegin, zEnd) into d_res ((zEnd - zBegin) * dims.x * dims.y floats, x-fastest). */
+int mi_fwn_vector_device(mi_mesh* m, const float* d_pts, int64_t n, float beta, int32_t skipFace,
+                         float* d_res, void* stream);
+int mi_fwn_grid_device(mi_mesh* m, const int32_t dims[3], const float xf[12], float beta,
+                       int32_t zBegin, int32_t zEnd, float* d_res, void* stream);
+int mi_sdf_grid_device(mi_mesh* m, const int32_t dims[3], const float xf[12],
+                       const mi_dist_options* opts, int32_t zBegin, int32_t zEnd,
+                       float* d_res, void* stream);
+
+/* ---------------------------------------------------------------- IPointsToMeshProjector */
+
+/* IPointsToMeshProjector::findProjections (source/MRMesh/MRPointsToMeshProjector.h:12-30;
+ * GPU source/MRCuda/MRCudaPointsToMeshProjector.cpp:52-125).  objXf / refObjXf may be NULL;
+ * a rigid refObjXf is folded into the point transform, a non-rigid one is applied to the
+ * tree boxes and triangles (createProjectionTransforms, source/MRMesh/MRMeshProject.cpp:105-121). */
+int mi_proj_find_projections(mi_mesh* m, const float* pts, int64_t n, const float* objXf,
+                             const float* refObjXf, float upDistLimitSq, float loDistLimitSq,
+                             mi_projection* res);
+/* IPointsToMeshProjector::projectionsHeapBytes (MRCudaPointsToMeshProjector.cpp:127-130) */
+size_t mi_proj_heap_bytes(size_t numProjections);
+/* device-resident variant: ptXf / treeXf are the already simplified transforms (or NULL) */
+int mi_proj_find_projections_device(mi_mesh* m, const float* d_pts, int64_t n, const float* ptXf,
+                                    const float* treeXf, float upDistLimitSq, float loDistLimitSq,
+                                    mi_projection* d_res, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,335 @@
+"""MI355X (gfx950) distance-field path: host mirror of MeshLib's GPU interfaces.
+
+Python surface of the C-ABI in ``include/meshlib_amd.h`` (``libmeshlib_amd.so``).  The
+class and method names mirror the reference interfaces this path drops in behind:
+
+* ``FastWindingNumber`` -- ``MR::IFastWindingNumber`` + ``MR::IFastWindingNumberByParts``
+  (source/MRMesh/MRFastWindingNumber.h:11-107), GPU class ``MR::Cuda::FastWindingNumber``
+  (source/MRCuda/MRCudaFastWindingNumber.h:17-38);
+* ``PointsToMeshProjector`` -- ``MR::IPointsToMeshProjector``
+  (source/MRMesh/MRPointsToMeshProjector.h:12-30), GPU class
+  ``MR::Cuda::PointsToMeshProjector`` (source/MRCuda/MRCudaPointsToMeshProjector.h:9-31);
+* ``DistanceToMeshOptions`` -- source/MRMesh/MRDistanceToMeshOptions.h:10-33.
+
+Errors surface as ``MeshLibAmdError`` carrying the library's message (the reference returns
+``Expected<void>`` strings).  There is no CPU fallback: without the built native library
+every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+from typing import Callable, Optional
+
+import numpy as np
+
+__all__ = ["Mesh", "FastWindingNumber", "PointsToMeshProjector", "DistanceToMeshOptions",
+           "MeshLibAmdError", "PROJ_DTYPE", "NODE_DTYPE", "DIPOLE_DTYPE", "lib", "lib_path",
+           "set_memory_limit", "FLT_MAX"]
+
+FLT_MAX = float(np.finfo(np.float32).max)
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libmeshlib_amd.so")
+
+PROJ_DTYPE = np.dtype([("face", "<i4"), ("proj", "<f4", 3), ("e", "<i4"), ("a", "<f4"),
+                       ("b", "<f4"), ("distSq", "<f4")])
+NODE_DTYPE = np.dtype([("min", "<f4", 3), ("max", "<f4", 3), ("l", "<i4"), ("r", "<i4")])
+DIPOLE_DTYPE = np.dtype([("pos", "<f4", 3), ("area", "<f4"), ("dirArea", "<f4", 3), ("rr", "<f4")])
+
+MI_OK, MI_E_INVALID, MI_E_HIP, MI_E_CANCELED, MI_E_NOMEM, MI_E_LIMIT = 0, -1, -2, -3, -4, -5
+
+
+class MeshLibAmdError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(msg)
+        self.code = code
+
+
+class _DistOpts(C.Structure):
+    _fields_ = [("minDistSq", C.c_float), ("maxDistSq", C.c_float), ("nullOutsideMinMax", C.c_int32),
+                ("windingNumberThreshold", C.c_float), ("windingNumberBeta", C.c_float)]
+
+
+_PROGRESS = C.CFUNCTYPE(C.c_int, C.c_float, C.c_void_p)
+_PART = C.CFUNCTYPE(C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_int32), C.c_int32, C.c_void_p)
+
+_lib = None
+
+
+def lib_path() -> str:
+    return _LIB_PATH
+
+
+def lib():
+    """Loads the native library; raises if it has not been built (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        raise MeshLibAmdError(MI_E_INVALID, f"native library missing: {_LIB_PATH} "
+                              "(build it with `python -m meshlib_amd.build`)")
+    L = C.CDLL(_LIB_PATH)
+    P, I32, I64, F, SZ = C.c_void_p, C.c_int32, C.c_int64, C.c_float, C.c_size_t
+    sig = {
+        "mi_last_error": (SZ, [C.c_char_p, SZ]),
+        "mi_set_memory_limit": (C.c_int, [SZ]),
+        "mi_mesh_create": (C.c_int, [P, I32, P, I32, C.POINTER(P)]),
+        "mi_mesh_destroy": (None, [P]),
+        "mi_mesh_num_nodes": (C.c_int, [P]),
+        "mi_mesh_get_tree": (C.c_int, [P, P, P]),
+        "mi_mesh_depth": (C.c_int, [P]),
+        "mi_build_tree_host": (C.c_int, [P, I32, P, I32, P, P, P]),
+        "mi_fwn_calc_from_vector": (C.c_int, [P, P, I64, F, I32, P, _PROGRESS, P]),
+        "mi_fwn_calc_self_intersections": (C.c_int, [P, F, P, _PROGRESS, P]),
+        "mi_fwn_calc_from_grid": (C.c_int, [P, P, P, F, P, _PROGRESS, P]),
+        "mi_fwn_calc_from_grid_with_distances": (C.c_int, [P, P, P, P, P, _PROGRESS, P]),
+        "mi_fwn_calc_from_grid_by_parts": (C.c_int, [P, _PART, P, P, P, F, I32, _PROGRESS, P]),
+        "mi_fwn_calc_from_grid_with_distances_by_parts": (C.c_int, [P, _PART, P, P, P, P, I32, _PROGRESS, P]),
+        "mi_fwn_vector_device": (C.c_int, [P, P, I64, F, I32, P, P]),
+        "mi_fwn_grid_device": (C.c_int, [P, P, P, F, I32, I32, P, P]),
+        "mi_sdf_grid_device": (C.c_int, [P, P, P, P, I32, I32, P, P]),
+        "mi_proj_find_projections": (C.c_int, [P, P, I64, P, P, F, F, P]),
+        "mi_proj_heap_bytes": (SZ, [SZ]),
+        "mi_proj_find_projections_device": (C.c_int, [P, P, I64, P, P, F, F, P, P]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(rc: int):
+    if rc != MI_OK:
+        buf = C.create_string_buffer(4096)
+        lib().mi_last_error(buf, 4096)
+        raise MeshLibAmdError(rc, buf.value.decode(errors="replace") or f"status {rc}")
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _xf(xf):
+    if xf is None:
+        return None
+    a = np.ascontiguousarray(xf, dtype=np.float32).reshape(-1)
+    if a.size == 16:      # 4x4 homogeneous matrix -> A rows + b
+        m = a.reshape(4, 4)
+        a = np.concatenate([m[:3, :3].reshape(-1), m[:3, 3]]).astype(np.float32)
+    if a.size != 12:
+        raise ValueError("affine transform must be float[12] (row-major A then b) or 4x4")
+    return a
+
+
+def _dims(d):
+    a = np.ascontiguousarray(d, dtype=np.int32).reshape(3)
+    return a
+
+
+def build_tree_host(verts, tris):
+    """Host-only tree + dipole build (no device): (nodes, dipoles, depth) in the reference layout."""
+    v = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1, 3)
+    t = np.ascontiguousarray(tris, dtype=np.int32).reshape(-1, 3)
+    n = max(2 * t.shape[0] - 1, 0)
+    nodes = np.zeros(n, NODE_DTYPE)
+    dips = np.zeros(n, DIPOLE_DTYPE)
+    depth = C.c_int32(0)
+    _check(lib().mi_build_tree_host(_p(v), v.shape[0], _p(t), t.shape[0], _p(nodes), _p(dips),
+                                    C.byref(depth)))
+    return nodes, dips, int(depth.value)
+
+
+def set_memory_limit(nbytes: int):
+    """Caps the device-memory budget of the chunked entry points (0 = default policy)."""
+    _check(lib().mi_set_memory_limit(int(nbytes)))
+
+
+@dataclass
+class DistanceToMeshOptions:
+    minDistSq: float = 0.0
+    maxDistSq: float = FLT_MAX
+    nullOutsideMinMax: bool = True
+    windingNumberThreshold: float = 0.5
+    windingNumberBeta: float = 2.0
+
+    def _c(self):
+        return _DistOpts(self.minDistSq, self.maxDistSq, 1 if self.nullOutsideMinMax else 0,
+                         self.windingNumberThreshold, self.windingNumberBeta)
+
+
+def _progress(cb: Optional[Callable[[float], bool]]):
+    if cb is None:
+        return _PROGRESS(0), None
+    def f(v, _u):
+        try:
+            return 1 if cb(float(v)) else 0
+        except Exception:   # a raising callback cancels
+            return 0
+    c = _PROGRESS(f)
+    return c, c
+
+
+class Mesh:
+    """Triangle mesh (float32 vertices, int32 triangles in getTriangulation order) with its
+    device-resident AABB tree + dipoles (built on construction, like Mesh::getAABBTree())."""
+
+    def __init__(self, verts, tris):
+        self.verts = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1, 3)
+        self.tris = np.ascontiguousarray(tris, dtype=np.int32).reshape(-1, 3)
+        h = C.c_void_p()
+        _check(lib().mi_mesh_create(_p(self.verts), self.verts.shape[0], _p(self.tris),
+                                    self.tris.shape[0], C.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib().mi_mesh_destroy(h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def num_faces(self) -> int:
+        return int(self.tris.shape[0])
+
+    def tree(self):
+        """AABB tree nodes + dipoles in the reference layout (AABBTree::nodes(), getDipoles())."""
+        n = lib().mi_mesh_num_nodes(self._h)
+        nodes = np.zeros(n, NODE_DTYPE)
+        dips = np.zeros(n, DIPOLE_DTYPE)
+        _check(lib().mi_mesh_get_tree(self._h, _p(nodes), _p(dips)))
+        return nodes, dips
+
+    def depth(self) -> int:
+        return int(lib().mi_mesh_depth(self._h))
+
+
+class FastWindingNumber:
+    """``MR::IFastWindingNumber`` / ``IFastWindingNumberByParts`` on the GPU."""
+
+    def __init__(self, mesh: Mesh):
+        self.mesh = mesh
+
+    def calcFromVector(self, points, beta: float, skipFace: int = -1, cb=None) -> np.ndarray:
+        pts = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+        res = np.empty(pts.shape[0], np.float32)
+        c, _keep = _progress(cb)
+        _check(lib().mi_fwn_calc_from_vector(self.mesh.handle, _p(pts), pts.shape[0], beta, skipFace,
+                                             _p(res), c, None))
+        return res
+
+    def calcSelfIntersections(self, beta: float, cb=None) -> np.ndarray:
+        res = np.zeros(self.mesh.num_faces(), np.uint8)
+        c, _keep = _progress(cb)
+        _check(lib().mi_fwn_calc_self_intersections(self.mesh.handle, beta, _p(res), c, None))
+        return res.astype(bool)
+
+    def calcFromGrid(self, dims, gridToMeshXf, beta: float, cb=None) -> np.ndarray:
+        d = _dims(dims)
+        res = np.empty(int(d[0]) * int(d[1]) * int(d[2]), np.float32)
+        c, _keep = _progress(cb)
+        _check(lib().mi_fwn_calc_from_grid(self.mesh.handle, _p(d), _p(_xf(gridToMeshXf)), beta,
+                                           _p(res), c, None))
+        return res
+
+    def calcFromGridWithDistances(self, dims, gridToMeshXf, options: DistanceToMeshOptions = None,
+                                  cb=None) -> np.ndarray:
+        d = _dims(dims)
+        o = (options or DistanceToMeshOptions())._c()
+        res = np.empty(int(d[0]) * int(d[1]) * int(d[2]), np.float32)
+        c, _keep = _progress(cb)
+        _check(lib().mi_fwn_calc_from_grid_with_distances(self.mesh.handle, _p(d), _p(_xf(gridToMeshXf)),
+                                                          C.byref(o), _p(res), c, None))
+        return res
+
+    @staticmethod
+    def _part_cb(resFunc):
+        def f(data, dims, zoff, _u):
+            try:
+                dx, dy, dz = dims[0], dims[1], dims[2]
+                arr = np.ctypeslib.as_array(data, shape=(dx * dy * dz,)).copy()
+                r = resFunc(arr, (dx, dy, dz), int(zoff))
+                return MI_OK if (r is None or r is True) else MI_E_CANCELED
+            except Exception:
+                return MI_E_CANCELED
+        return _PART(f)
+
+    def calcFromGridByParts(self, resFunc, dims, gridToMeshXf, beta: float, layerOverlap: int,
+                            cb=None):
+        d = _dims(dims)
+        part = self._part_cb(resFunc)
+        c, _keep = _progress(cb)
+        _check(lib().mi_fwn_calc_from_grid_by_parts(self.mesh.handle, part, None, _p(d),
+                                                    _p(_xf(gridToMeshXf)), beta, layerOverlap, c, None))
+
+    def calcFromGridWithDistancesByParts(self, resFunc, dims, gridToMeshXf,
+                                         options: DistanceToMeshOptions, layerOverlap: int, cb=None):
+        d = _dims(dims)
+        o = (options or DistanceToMeshOptions())._c()
+        part = self._part_cb(resFunc)
+        c, _keep = _progress(cb)
+        _check(lib().mi_fwn_calc_from_grid_with_distances_by_parts(
+            self.mesh.handle, part, None, _p(d), _p(_xf(gridToMeshXf)), C.byref(o), layerOverlap, c, None))
+
+    # device-resident forms (pointers to HBM, e.g. torch tensors' data_ptr())
+    def sdf_grid_device(self, dims, xf, options, d_out: int, zBegin=0, zEnd=None, stream: int = 0):
+        d = _dims(dims)
+        o = (options or DistanceToMeshOptions())._c()
+        zEnd = int(d[2]) if zEnd is None else zEnd
+        _check(lib().mi_sdf_grid_device(self.mesh.handle, _p(d), _p(_xf(xf)), C.byref(o), zBegin, zEnd,
+                                        C.c_void_p(d_out), C.c_void_p(stream or None)))
+
+    def fwn_grid_device(self, dims, xf, beta, d_out: int, zBegin=0, zEnd=None, stream: int = 0):
+        d = _dims(dims)
+        zEnd = int(d[2]) if zEnd is None else zEnd
+        _check(lib().mi_fwn_grid_device(self.mesh.handle, _p(d), _p(_xf(xf)), beta, zBegin, zEnd,
+                                        C.c_void_p(d_out), C.c_void_p(stream or None)))
+
+    def fwn_vector_device(self, d_pts: int, n: int, beta: float, d_out: int, skipFace: int = -1,
+                          stream: int = 0):
+        _check(lib().mi_fwn_vector_device(self.mesh.handle, C.c_void_p(d_pts), n, beta, skipFace,
+                                          C.c_void_p(d_out), C.c_void_p(stream or None)))
+
+
+class PointsToMeshProjector:
+    """``MR::IPointsToMeshProjector`` on the GPU."""
+
+    def __init__(self):
+        self.mesh: Optional[Mesh] = None
+
+    def updateMeshData(self, mesh: Optional[Mesh]):
+        self.mesh = mesh
+
+    def findProjections(self, points, objXf=None, refObjXf=None, upDistLimitSq: float = FLT_MAX,
+                        loDistLimitSq: float = 0.0) -> np.ndarray:
+        if self.mesh is None:
+            raise MeshLibAmdError(MI_E_INVALID, "updateMeshData was not called")
+        pts = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+        res = np.zeros(pts.shape[0], PROJ_DTYPE)
+        _check(lib().mi_proj_find_projections(self.mesh.handle, _p(pts), pts.shape[0], _p(_xf(objXf)),
+                                              _p(_xf(refObjXf)), upDistLimitSq, loDistLimitSq, _p(res)))
+        return res
+
+    def projectionsHeapBytes(self, numProjections: int) -> int:
+        return int(lib().mi_proj_heap_bytes(numProjections))
+
+    def find_projections_device(self, d_pts: int, n: int, d_out: int, ptXf=None, treeXf=None,
+                                upDistLimitSq: float = FLT_MAX, loDistLimitSq: float = 0.0,
+                                stream: int = 0):
+        if self.mesh is None:
+            raise MeshLibAmdError(MI_E_INVALID, "updateMeshData was not called")
+        _check(lib().mi_proj_find_projections_device(self.mesh.handle, C.c_void_p(d_pts), n,
+                                                     _p(_xf(ptXf)), _p(_xf(treeXf)), upDistLimitSq,
+                                                     loDistLimitSq, C.c_void_p(d_out),
+                                                     C.c_void_p(stream or None)))

@@ -1,0 +1,637 @@
+// mi_capi.cpp -- the C-ABI (include/meshlib_amd.h): mesh upload, chunked host-pointer
+// entry points mirroring the reference's synchronous GPU classes, device-resident entry
+// points, error strings and the memory budget.
+#include "meshlib_amd.h"
+
+#include "mi_kernels.hpp"
+#include "mi_tree.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_err;
+std::atomic<size_t> g_memLimit{ 0 };
+
+int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define MI_HIP(expr)                                                                              \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            return fail(MI_E_HIP, std::string("AMD GPU error: ") + hipGetErrorString(e_) + " (" + \
+                                      #expr + ")");                                               \
+    } while (0)
+
+inline size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
+
+// getCudaSafeMemoryLimit (source/MRCuda/MRCudaBasic.cpp:75-92): 80 % of (free - 128 MiB),
+// optionally capped by mi_set_memory_limit
+size_t memBudget()
+{
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) return 0;
+    const size_t reserve = size_t(128) << 20;
+    const size_t avail = fr > reserve ? fr - reserve : 0;
+    size_t lim = (size_t)((float)avail * 0.80f);
+    const size_t u = g_memLimit.load();
+    if (u && u < lim) lim = u;
+    return lim;
+}
+
+// RAII device buffer
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    hipError_t alloc(size_t b)
+    {
+        if (p) { (void)hipFree(p); p = nullptr; bytes = 0; }
+        if (b == 0) return hipSuccess;
+        hipError_t e = hipMalloc(&p, b);
+        if (e == hipSuccess) bytes = b;
+        return e;
+    }
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+bool report(mi_progress_fn cb, void* user, float v) { return !cb || cb(v, user) != 0; }
+
+// ---------------------------------------------------------------- affine helpers (host)
+
+struct M3 { float r[3][3]; };
+
+float dot3(const float* a, const float* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+// isRigid (source/MRMesh/MRMatrix3Decompose.h:27-39): Gram-Schmidt QR of the columns
+// (MRMatrix3.h:272-295), rigid when |r_ii| are within 10 eps of 1
+bool isRigid(const float* xf)
+{
+    float a0[3] = { xf[0], xf[3], xf[6] }, a1[3] = { xf[1], xf[4], xf[7] }, a2[3] = { xf[2], xf[5], xf[8] };
+    auto len = [](const float* v) { return std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); };
+    auto scaled = [](const float* v, float d, float* o) {
+        const float inv = 1 / d;
+        for (int i = 0; i < 3; ++i) o[i] = v[i] * inv;
+    };
+    const float r00 = len(a0);
+    float e0[3] = { 0, 0, 0 };
+    if (r00 > 0) scaled(a0, r00, e0);
+    const float r01 = dot3(e0, a1), r02 = dot3(e0, a2);
+    for (int i = 0; i < 3; ++i) a1[i] -= r01 * e0[i];
+    const float r11 = len(a1);
+    float e1[3] = { 0, 0, 0 };
+    if (r11 > 0) scaled(a1, r11, e1);
+    const float r12 = dot3(e1, a2);
+    for (int i = 0; i < 3; ++i) a2[i] -= r02 * e0[i] + r12 * e1[i];
+    const float r22 = len(a2);
+    const float eps = 10 * FLT_EPSILON;
+    return std::fabs(std::fabs(r00) - 1) <= eps && std::fabs(std::fabs(r11) - 1) <= eps &&
+           std::fabs(std::fabs(r22) - 1) <= eps;
+}
+
+// AffineXf::inverse (source/MRMesh/MRAffineXf.h:51-57), Matrix3::inverse (MRMatrix3.h:236-247)
+void xfInverse(const float* xf, float* out)
+{
+    const float* x = xf; const float* y = xf + 3; const float* z = xf + 6;
+    const float det = x[0] * (y[1] * z[2] - y[2] * z[1]) - x[1] * (y[0] * z[2] - y[2] * z[0]) +
+                      x[2] * (y[0] * z[1] - y[1] * z[0]);
+    float A[9] = { 0, 0, 0, 0, 0, 0, 0, 0, 0 };
+    if (det != 0) {
+        const float adj[9] = { y[1] * z[2] - y[2] * z[1], x[2] * z[1] - x[1] * z[2], x[1] * y[2] - x[2] * y[1],
+                               y[2] * z[0] - y[0] * z[2], x[0] * z[2] - x[2] * z[0], x[2] * y[0] - x[0] * y[2],
+                               y[0] * z[1] - y[1] * z[0], x[1] * z[0] - x[0] * z[1], x[0] * y[1] - x[1] * y[0] };
+        const float inv = 1 / det;
+        for (int i = 0; i < 9; ++i) A[i] = inv * adj[i];
+    }
+    for (int i = 0; i < 9; ++i) out[i] = A[i];
+    for (int i = 0; i < 3; ++i) out[9 + i] = -dot3(A + 3 * i, xf + 9);
+}
+
+// u * v = { u.A * v.A, u.A * v.b + u.b } (source/MRMesh/MRAffineXf.h:61-62, MRMatrix3.h:133-140)
+void xfCompose(const float* u, const float* v, float* out)
+{
+    float r[12];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            const float col[3] = { v[j], v[3 + j], v[6 + j] };
+            r[3 * i + j] = dot3(u + 3 * i, col);
+        }
+    for (int i = 0; i < 3; ++i) r[9 + i] = dot3(u + 3 * i, v + 9) + u[9 + i];
+    std::memcpy(out, r, sizeof(r));
+}
+
+mi::Xf toXf(const float* xf)
+{
+    mi::Xf x;
+    std::memcpy(x.m, xf, sizeof(x.m));
+    return x;
+}
+
+const float kIdentityXf[12] = { 1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0 };
+
+} // namespace
+
+struct mi_mesh {
+    int device = 0;
+    int32_t nv = 0, nf = 0;
+    mi::HostTree tree;
+    int stackD = 8;
+    DevBuf arena;
+    mi::MeshDev dev{};
+    float sortLo[3] = { 0, 0, 0 }, sortHi[3] = { 0, 0, 0 };
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    DevBuf ws;   // sort workspace of the device-resident entry points (grow-only)
+    ~mi_mesh() { if (stream) (void)hipStreamDestroy(stream); }
+};
+
+namespace {
+
+int ensureWorkspace(mi_mesh* m, size_t bytes)
+{
+    if (m->ws.bytes >= bytes) return MI_OK;
+    MI_HIP(m->ws.alloc(bytes));
+    return MI_OK;
+}
+
+hipStream_t pick(mi_mesh* m, void* s) { return s ? static_cast<hipStream_t>(s) : m->stream; }
+
+int checkGridArgs(const int32_t* dims, const float* xf)
+{
+    if (!dims || !xf) return fail(MI_E_INVALID, "dims and xf are required");
+    if (dims[0] < 0 || dims[1] < 0 || dims[2] < 0) return fail(MI_E_INVALID, "negative grid dims");
+    if ((int64_t)dims[0] * dims[1] > INT32_MAX) return fail(MI_E_LIMIT, "grid layer too large");
+    return MI_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+size_t mi_last_error(char* buf, size_t len)
+{
+    if (buf && len) {
+        const size_t n = std::min(len - 1, g_err.size());
+        std::memcpy(buf, g_err.data(), n);
+        buf[n] = 0;
+    }
+    return g_err.size();
+}
+
+int mi_set_memory_limit(size_t bytes)
+{
+    g_memLimit.store(bytes);
+    return MI_OK;
+}
+
+int mi_mesh_create(const float* verts, int32_t nv, const int32_t* tris, int32_t nf, mi_mesh** out)
+{
+    if (!out) return fail(MI_E_INVALID, "out is null");
+    *out = nullptr;
+    if (nv < 0 || nf < 0 || (nv > 0 && !verts) || (nf > 0 && !tris))
+        return fail(MI_E_INVALID, "invalid mesh arrays");
+    for (int64_t i = 0; i < 3 * (int64_t)nf; ++i)
+        if (tris[i] < 0 || tris[i] >= nv) return fail(MI_E_INVALID, "triangle vertex index out of range");
+    mi_mesh* m = new (std::nothrow) mi_mesh();
+    if (!m) return fail(MI_E_NOMEM, "out of host memory");
+    m->nv = nv;
+    m->nf = nf;
+    auto cleanup = [&](int code) { delete m; return code; };
+    if (hipGetDevice(&m->device) != hipSuccess) return cleanup(fail(MI_E_HIP, "AMD GPU error: no device"));
+    if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess)
+        return cleanup(fail(MI_E_HIP, "AMD GPU error: stream creation failed"));
+    try {
+        mi::buildHostTree(verts, nv, tris, nf, m->tree);
+    } catch (const std::bad_alloc&) {
+        return cleanup(fail(MI_E_NOMEM, "out of host memory building the tree"));
+    }
+    m->stackD = mi::stackDepthFor(m->tree.depth);
+    if (m->stackD == 0) return cleanup(fail(MI_E_LIMIT, "tree deeper than 32 levels"));
+    mi::FlatTree ft;
+    mi::flattenTree(m->tree, verts, tris, ft);
+    const size_t bDnode = align256(ft.dnode.size() * 4), bTri = align256(ft.tri.size() * 4);
+    const size_t bDint = align256(ft.dint.size() * 4), bDleaf = align256(ft.dleaf.size() * 4);
+    const size_t total = bDnode + bTri + bDint + bDleaf;
+    if (total) {
+        if (m->arena.alloc(total) != hipSuccess)
+            return cleanup(fail(MI_E_HIP, "AMD GPU error: cannot allocate the tree arena"));
+        char* base = m->arena.as<char>();
+        hipError_t e = hipSuccess;
+        if (!ft.dnode.empty()) e = hipMemcpy(base, ft.dnode.data(), ft.dnode.size() * 4, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(base + bDnode, ft.tri.data(), ft.tri.size() * 4, hipMemcpyHostToDevice);
+        if (e == hipSuccess && !ft.dint.empty())
+            e = hipMemcpy(base + bDnode + bTri, ft.dint.data(), ft.dint.size() * 4, hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(base + bDnode + bTri + bDint, ft.dleaf.data(), ft.dleaf.size() * 4, hipMemcpyHostToDevice);
+        if (e != hipSuccess) return cleanup(fail(MI_E_HIP, std::string("AMD GPU error: ") + hipGetErrorString(e)));
+        m->dev.dnode = reinterpret_cast<const float4*>(base);
+        m->dev.tri = reinterpret_cast<const float4*>(base + bDnode);
+        m->dev.dint = reinterpret_cast<const float4*>(base + bDnode + bTri);
+        m->dev.dleaf = reinterpret_cast<const float4*>(base + bDnode + bTri + bDint);
+    }
+    for (int i = 0; i < 3; ++i) {
+        m->dev.rootMin[i] = ft.rootBox.min[i];
+        m->dev.rootMax[i] = ft.rootBox.max[i];
+        const float c = 0.5f * (ft.rootBox.min[i] + ft.rootBox.max[i]);
+        const float h = 0.75f * (ft.rootBox.max[i] - ft.rootBox.min[i]);
+        m->sortLo[i] = c - h;
+        m->sortHi[i] = c + h;
+    }
+    m->dev.rootRef = ft.rootRef;
+    m->dev.numFaces = nf;
+    *out = m;
+    return MI_OK;
+}
+
+void mi_mesh_destroy(mi_mesh* m)
+{
+    if (!m) return;
+    (void)hipSetDevice(m->device);
+    delete m;
+}
+
+int mi_mesh_num_nodes(const mi_mesh* m) { return m ? (int)m->tree.nodes.size() : 0; }
+
+int mi_mesh_depth(const mi_mesh* m) { return m ? m->tree.depth : 0; }
+
+int mi_build_tree_host(const float* verts, int32_t nv, const int32_t* tris, int32_t nf, mi_tree_node* nodes,
+                       mi_dipole* dipoles, int32_t* depth)
+{
+    if (nv < 0 || nf < 0 || (nv > 0 && !verts) || (nf > 0 && !tris))
+        return fail(MI_E_INVALID, "invalid mesh arrays");
+    for (int64_t i = 0; i < 3 * (int64_t)nf; ++i)
+        if (tris[i] < 0 || tris[i] >= nv) return fail(MI_E_INVALID, "triangle vertex index out of range");
+    mi::HostTree t;
+    try {
+        mi::buildHostTree(verts, nv, tris, nf, t);
+    } catch (const std::bad_alloc&) {
+        return fail(MI_E_NOMEM, "out of host memory building the tree");
+    }
+    if (nodes && !t.nodes.empty()) std::memcpy(nodes, t.nodes.data(), t.nodes.size() * sizeof(mi_tree_node));
+    if (dipoles && !t.dipoles.empty())
+        std::memcpy(dipoles, t.dipoles.data(), t.dipoles.size() * sizeof(mi_dipole));
+    if (depth) *depth = t.depth;
+    return MI_OK;
+}
+
+int mi_mesh_get_tree(const mi_mesh* m, mi_tree_node* nodes, mi_dipole* dipoles)
+{
+    if (!m) return fail(MI_E_INVALID, "mesh is null");
+    static_assert(sizeof(mi_tree_node) == sizeof(mi::TreeNode), "node layout");
+    static_assert(sizeof(mi_dipole) == sizeof(mi::Dipole), "dipole layout");
+    if (nodes && !m->tree.nodes.empty())
+        std::memcpy(nodes, m->tree.nodes.data(), m->tree.nodes.size() * sizeof(mi_tree_node));
+    if (dipoles && !m->tree.dipoles.empty())
+        std::memcpy(dipoles, m->tree.dipoles.data(), m->tree.dipoles.size() * sizeof(mi_dipole));
+    return MI_OK;
+}
+
+// ---------------------------------------------------------------- device-resident entry points
+
+int mi_sdf_grid_device(mi_mesh* m, const int32_t dims[3], const float xf[12], const mi_dist_options* opts,
+                       int32_t zBegin, int32_t zEnd, float* d_res, void* stream)
+{
+    if (!m || !opts) return fail(MI_E_INVALID, "mesh and options are required");
+    if (int rc = checkGridArgs(dims, xf)) return rc;
+    if (zBegin < 0 || zEnd > dims[2] || zBegin > zEnd) return fail(MI_E_INVALID, "bad z-layer range");
+    if (zEnd > zBegin && (int64_t)dims[0] * dims[1] > 0 && !d_res) return fail(MI_E_INVALID, "output is null");
+    MI_HIP(hipSetDevice(m->device));
+    mi::DistOpts o{ opts->minDistSq, opts->maxDistSq, opts->nullOutsideMinMax, opts->windingNumberThreshold,
+                    opts->windingNumberBeta };
+    MI_HIP(mi::launchSdfGrid(m->dev, dims, toXf(xf), o, zBegin, zEnd, d_res, m->stackD, pick(m, stream)));
+    return MI_OK;
+}
+
+int mi_fwn_grid_device(mi_mesh* m, const int32_t dims[3], const float xf[12], float beta, int32_t zBegin,
+                       int32_t zEnd, float* d_res, void* stream)
+{
+    if (!m) return fail(MI_E_INVALID, "mesh is null");
+    if (int rc = checkGridArgs(dims, xf)) return rc;
+    if (zBegin < 0 || zEnd > dims[2] || zBegin > zEnd) return fail(MI_E_INVALID, "bad z-layer range");
+    if (zEnd > zBegin && (int64_t)dims[0] * dims[1] > 0 && !d_res) return fail(MI_E_INVALID, "output is null");
+    MI_HIP(hipSetDevice(m->device));
+    MI_HIP(mi::launchFwnGrid(m->dev, dims, toXf(xf), beta, zBegin, zEnd, d_res, m->stackD, pick(m, stream)));
+    return MI_OK;
+}
+
+int mi_fwn_vector_device(mi_mesh* m, const float* d_pts, int64_t n, float beta, int32_t skipFace, float* d_res,
+                         void* stream)
+{
+    if (!m || n < 0 || (n > 0 && (!d_pts || !d_res))) return fail(MI_E_INVALID, "invalid arguments");
+    if (n > INT32_MAX) return fail(MI_E_LIMIT, "more than 2^31-1 points in one call");
+    if (n == 0) return MI_OK;
+    std::lock_guard<std::mutex> lk(m->mu);
+    MI_HIP(hipSetDevice(m->device));
+    hipStream_t s = pick(m, stream);
+    if (int rc = ensureWorkspace(m, mi::sortWorkspaceBytes(n))) return rc;
+    const uint32_t* perm = nullptr;
+    MI_HIP(mi::sortPointsMorton(d_pts, n, m->sortLo, m->sortHi, m->ws.p, m->ws.bytes, &perm, s));
+    MI_HIP(mi::launchFwnPoints(m->dev, d_pts, perm, n, beta, skipFace, d_res, m->stackD, s));
+    return MI_OK;
+}
+
+int mi_proj_find_projections_device(mi_mesh* m, const float* d_pts, int64_t n, const float* ptXf,
+                                    const float* treeXf, float up, float lo, mi_projection* d_res, void* stream)
+{
+    if (!m || n < 0 || (n > 0 && (!d_pts || !d_res))) return fail(MI_E_INVALID, "invalid arguments");
+    if (n > INT32_MAX) return fail(MI_E_LIMIT, "more than 2^31-1 points in one call");
+    if (n == 0) return MI_OK;
+    std::lock_guard<std::mutex> lk(m->mu);
+    MI_HIP(hipSetDevice(m->device));
+    hipStream_t s = pick(m, stream);
+    if (int rc = ensureWorkspace(m, mi::sortWorkspaceBytes(n))) return rc;
+    const uint32_t* perm = nullptr;
+    MI_HIP(mi::sortPointsMorton(d_pts, n, m->sortLo, m->sortHi, m->ws.p, m->ws.bytes, &perm, s));
+    mi::Xf px, tx;
+    if (ptXf) px = toXf(ptXf);
+    if (treeXf) tx = toXf(treeXf);
+    MI_HIP(mi::launchProjections(m->dev, d_pts, perm, n, ptXf ? &px : nullptr, treeXf ? &tx : nullptr, up, lo,
+                                 reinterpret_cast<mi::ProjRec*>(d_res), m->stackD, s));
+    return MI_OK;
+}
+
+// ---------------------------------------------------------------- host-pointer entry points
+
+static int gridHost(mi_mesh* m, const int32_t dims[3], const float xf[12], const mi_dist_options* opts, float beta,
+                    float* res, mi_progress_fn cb, void* user)
+{
+    if (!m) return fail(MI_E_INVALID, "mesh is null");
+    if (int rc = checkGridArgs(dims, xf)) return rc;
+    const size_t layer = (size_t)dims[0] * dims[1];
+    const size_t total = layer * dims[2];
+    if (total && !res) return fail(MI_E_INVALID, "output is null");
+    std::lock_guard<std::mutex> lk(m->mu);
+    MI_HIP(hipSetDevice(m->device));
+    if (!report(cb, user, 0.5f)) return fail(MI_E_CANCELED, "Operation was canceled");
+    if (total == 0) return MI_OK;
+    // maxBufferSizeAlignedByBlock (source/MRCuda/MRCudaBasic.cpp:103-107): whole z-layers
+    const size_t layers = std::min(memBudget() / sizeof(float) / layer, (size_t)dims[2]);
+    if (layers == 0) return fail(MI_E_LIMIT, "one z-layer does not fit in the device memory budget");
+    DevBuf out;
+    MI_HIP(out.alloc(layers * layer * sizeof(float)));
+    if (!report(cb, user, 0.6f)) return fail(MI_E_CANCELED, "Operation was canceled");
+    const int chunks = (int)((dims[2] + layers - 1) / layers);
+    const mi::Xf x = toXf(xf);
+    for (int c = 0; c < chunks; ++c) {
+        const int32_t z0 = (int32_t)(c * layers), z1 = (int32_t)std::min<size_t>(dims[2], (c + 1) * layers);
+        if (opts) {
+            mi::DistOpts o{ opts->minDistSq, opts->maxDistSq, opts->nullOutsideMinMax,
+                            opts->windingNumberThreshold, opts->windingNumberBeta };
+            MI_HIP(mi::launchSdfGrid(m->dev, dims, x, o, z0, z1, out.as<float>(), m->stackD, m->stream));
+        } else {
+            MI_HIP(mi::launchFwnGrid(m->dev, dims, x, beta, z0, z1, out.as<float>(), m->stackD, m->stream));
+        }
+        MI_HIP(hipMemcpyAsync(res + (size_t)z0 * layer, out.p, (size_t)(z1 - z0) * layer * sizeof(float),
+                              hipMemcpyDeviceToHost, m->stream));
+        MI_HIP(hipStreamSynchronize(m->stream));
+        if (!report(cb, user, 0.6f + 0.4f * (float)(c + 1) / (float)chunks))
+            return fail(MI_E_CANCELED, "Operation was canceled");
+    }
+    return MI_OK;
+}
+
+int mi_fwn_calc_from_grid(mi_mesh* m, const int32_t dims[3], const float xf[12], float beta, float* res,
+                          mi_progress_fn cb, void* user)
+{
+    return gridHost(m, dims, xf, nullptr, beta, res, cb, user);
+}
+
+int mi_fwn_calc_from_grid_with_distances(mi_mesh* m, const int32_t dims[3], const float xf[12],
+                                         const mi_dist_options* opts, float* res, mi_progress_fn cb, void* user)
+{
+    if (!opts) return fail(MI_E_INVALID, "options are required");
+    return gridHost(m, dims, xf, opts, 0.f, res, cb, user);
+}
+
+// by-parts: the GPU computes chunk k+1 while the caller's callback consumes chunk k
+// (cudaPipeline, source/MRCuda/MRCudaPipeline.hpp:9-48, on two HIP streams)
+static int gridByParts(mi_mesh* m, mi_grid_part_fn fn, void* fnUser, const int32_t dims[3], const float xf[12],
+                       const mi_dist_options* opts, float beta, int32_t layerOverlap, mi_progress_fn cb, void* user)
+{
+    if (!m || !fn) return fail(MI_E_INVALID, "mesh and callback are required");
+    if (int rc = checkGridArgs(dims, xf)) return rc;
+    if (layerOverlap < 0) return fail(MI_E_INVALID, "negative layer overlap");
+    const size_t layer = (size_t)dims[0] * dims[1];
+    const size_t total = layer * dims[2];
+    std::lock_guard<std::mutex> lk(m->mu);
+    MI_HIP(hipSetDevice(m->device));
+    if (!report(cb, user, 0.5f)) return fail(MI_E_CANCELED, "Operation was canceled");
+    if (total == 0) return MI_OK;
+    // two device buffers + two pinned host buffers, each of `chunkLayers` whole z-layers
+    const size_t chunkLayers = std::min(memBudget() / sizeof(float) / layer / 2, (size_t)dims[2]);
+    if (chunkLayers == 0) return fail(MI_E_LIMIT, "one z-layer does not fit in the device memory budget");
+    // splitByChunks (source/MRMesh/MRChunkIterator.cpp:29-53) in units of layers
+    size_t step = chunkLayers, nChunks = 1;
+    if ((size_t)dims[2] > chunkLayers) {
+        if ((size_t)layerOverlap >= chunkLayers)
+            return fail(MI_E_LIMIT, "layer overlap does not fit in the device memory budget");
+        step = chunkLayers - (size_t)layerOverlap;
+        nChunks = ((size_t)dims[2] - (size_t)layerOverlap + step - 1) / step;
+    }
+    DevBuf dbuf[2];
+    struct Host {
+        float* hbuf[2] = { nullptr, nullptr };
+        hipStream_t side = nullptr, main = nullptr;
+        hipEvent_t ev[2] = { nullptr, nullptr };
+        ~Host()
+        {
+            if (main) (void)hipStreamSynchronize(main);
+            for (int i = 0; i < 2; ++i) {
+                if (hbuf[i]) (void)hipHostFree(hbuf[i]);
+                if (ev[i]) (void)hipEventDestroy(ev[i]);
+            }
+            if (side) { (void)hipStreamSynchronize(side); (void)hipStreamDestroy(side); }
+        }
+    } h;
+    float** hbuf = h.hbuf;
+    hipEvent_t* ev = h.ev;
+    MI_HIP(hipStreamCreateWithFlags(&h.side, hipStreamNonBlocking));
+    h.main = m->stream;
+    hipStream_t st[2] = { m->stream, h.side };
+    for (int i = 0; i < 2; ++i) {
+        MI_HIP(dbuf[i].alloc(chunkLayers * layer * sizeof(float)));
+        MI_HIP(hipHostMalloc(reinterpret_cast<void**>(&hbuf[i]), chunkLayers * layer * sizeof(float),
+                             hipHostMallocDefault));
+        MI_HIP(hipEventCreateWithFlags(&ev[i], hipEventDisableTiming));
+    }
+    if (!report(cb, user, 0.6f)) return fail(MI_E_CANCELED, "Operation was canceled");
+    const mi::Xf x = toXf(xf);
+    auto chunkRange = [&](size_t c, int32_t& z0, int32_t& z1) {
+        z0 = (int32_t)(c * step);
+        z1 = (int32_t)std::min<size_t>(dims[2], z0 + chunkLayers);
+    };
+    auto launch = [&](size_t c) -> int {
+        const int b = (int)(c & 1);
+        int32_t z0, z1;
+        chunkRange(c, z0, z1);
+        if (opts) {
+            mi::DistOpts o{ opts->minDistSq, opts->maxDistSq, opts->nullOutsideMinMax,
+                            opts->windingNumberThreshold, opts->windingNumberBeta };
+            MI_HIP(mi::launchSdfGrid(m->dev, dims, x, o, z0, z1, dbuf[b].as<float>(), m->stackD, st[b]));
+        } else {
+            MI_HIP(mi::launchFwnGrid(m->dev, dims, x, beta, z0, z1, dbuf[b].as<float>(), m->stackD, st[b]));
+        }
+        MI_HIP(hipMemcpyAsync(hbuf[b], dbuf[b].p, (size_t)(z1 - z0) * layer * sizeof(float),
+                              hipMemcpyDeviceToHost, st[b]));
+        MI_HIP(hipEventRecord(ev[b], st[b]));
+        return MI_OK;
+    };
+    if (int rc = launch(0)) return rc;
+    for (size_t c = 0; c < nChunks; ++c) {
+        const int b = (int)(c & 1);
+        MI_HIP(hipEventSynchronize(ev[b]));
+        if (c + 1 < nChunks)
+            if (int rc = launch(c + 1)) return rc;
+        int32_t z0, z1;
+        chunkRange(c, z0, z1);
+        const int32_t pd[3] = { dims[0], dims[1], z1 - z0 };
+        const int rc = fn(hbuf[b], pd, z0, fnUser);
+        if (rc != MI_OK) {
+            (void)hipStreamSynchronize(st[0]);
+            (void)hipStreamSynchronize(st[1]);
+            return fail(rc, "grid part callback failed");
+        }
+        if (!report(cb, user, 0.6f + 0.4f * (float)(c + 1) / (float)nChunks)) {
+            (void)hipStreamSynchronize(st[0]);
+            (void)hipStreamSynchronize(st[1]);
+            return fail(MI_E_CANCELED, "Operation was canceled");
+        }
+    }
+    return MI_OK;
+}
+
+int mi_fwn_calc_from_grid_by_parts(mi_mesh* m, mi_grid_part_fn fn, void* fnUser, const int32_t dims[3],
+                                   const float xf[12], float beta, int32_t layerOverlap, mi_progress_fn cb,
+                                   void* user)
+{
+    return gridByParts(m, fn, fnUser, dims, xf, nullptr, beta, layerOverlap, cb, user);
+}
+
+int mi_fwn_calc_from_grid_with_distances_by_parts(mi_mesh* m, mi_grid_part_fn fn, void* fnUser,
+                                                  const int32_t dims[3], const float xf[12],
+                                                  const mi_dist_options* opts, int32_t layerOverlap,
+                                                  mi_progress_fn cb, void* user)
+{
+    if (!opts) return fail(MI_E_INVALID, "options are required");
+    return gridByParts(m, fn, fnUser, dims, xf, opts, 0.f, layerOverlap, cb, user);
+}
+
+} // extern "C"
+
+// per-point chunked pipeline shared by calcFromVector and findProjections
+template <class LaunchFn>
+static int pointsHost(mi_mesh* m, const float* pts, int64_t n, size_t outElem, void* res, LaunchFn&& launchFn,
+                      mi_progress_fn cb, void* user)
+{
+    std::lock_guard<std::mutex> lk(m->mu);
+    MI_HIP(hipSetDevice(m->device));
+    if (!report(cb, user, 0.5f)) return fail(MI_E_CANCELED, "Operation was canceled");
+    if (n == 0) return MI_OK;
+    // maxBufferSize (source/MRCuda/MRCudaBasic.cpp:94-97) with the sort workspace counted
+    const size_t perElem = 12 + outElem + 20;
+    size_t chunk = std::min<size_t>(memBudget() / perElem, (size_t)n);
+    chunk = std::min<size_t>(chunk, (size_t)INT32_MAX);
+    if (chunk == 0) return fail(MI_E_LIMIT, "no point fits in the device memory budget");
+    DevBuf dpts, dres, ws;
+    MI_HIP(dpts.alloc(chunk * 12));
+    MI_HIP(dres.alloc(chunk * outElem));
+    MI_HIP(ws.alloc(mi::sortWorkspaceBytes((int64_t)chunk)));
+    if (!report(cb, user, 0.6f)) return fail(MI_E_CANCELED, "Operation was canceled");
+    const size_t chunks = ((size_t)n + chunk - 1) / chunk;
+    for (size_t c = 0; c < chunks; ++c) {
+        const size_t off = c * chunk, sz = std::min(chunk, (size_t)n - off);
+        MI_HIP(hipMemcpyAsync(dpts.p, pts + 3 * off, sz * 12, hipMemcpyHostToDevice, m->stream));
+        const uint32_t* perm = nullptr;
+        MI_HIP(mi::sortPointsMorton(dpts.as<float>(), (int64_t)sz, m->sortLo, m->sortHi, ws.p, ws.bytes, &perm,
+                                    m->stream));
+        MI_HIP(launchFn(dpts.as<float>(), perm, (int64_t)sz, dres.p));
+        MI_HIP(hipMemcpyAsync(static_cast<char*>(res) + off * outElem, dres.p, sz * outElem,
+                              hipMemcpyDeviceToHost, m->stream));
+        MI_HIP(hipStreamSynchronize(m->stream));
+        if (!report(cb, user, 0.6f + 0.4f * (float)(c + 1) / (float)chunks))
+            return fail(MI_E_CANCELED, "Operation was canceled");
+    }
+    return MI_OK;
+}
+
+extern "C" {
+
+int mi_fwn_calc_from_vector(mi_mesh* m, const float* pts, int64_t n, float beta, int32_t skipFace, float* res,
+                            mi_progress_fn cb, void* user)
+{
+    if (!m || n < 0 || (n > 0 && (!pts || !res))) return fail(MI_E_INVALID, "invalid arguments");
+    return pointsHost(m, pts, n, sizeof(float), res,
+                      [&](const float* dp, const uint32_t* perm, int64_t sz, void* dr) {
+                          return mi::launchFwnPoints(m->dev, dp, perm, sz, beta, skipFace, static_cast<float*>(dr),
+                                                     m->stackD, m->stream);
+                      },
+                      cb, user);
+}
+
+int mi_fwn_calc_self_intersections(mi_mesh* m, float beta, uint8_t* res, mi_progress_fn cb, void* user)
+{
+    if (!m || (m->nf > 0 && !res)) return fail(MI_E_INVALID, "invalid arguments");
+    std::lock_guard<std::mutex> lk(m->mu);
+    MI_HIP(hipSetDevice(m->device));
+    if (!report(cb, user, 0.5f)) return fail(MI_E_CANCELED, "Operation was canceled");
+    if (m->nf == 0) return MI_OK;
+    DevBuf out;
+    MI_HIP(out.alloc((size_t)m->nf));
+    MI_HIP(mi::launchSelfIntersections(m->dev, beta, out.as<uint8_t>(), m->stackD, m->stream));
+    MI_HIP(hipMemcpyAsync(res, out.p, (size_t)m->nf, hipMemcpyDeviceToHost, m->stream));
+    MI_HIP(hipStreamSynchronize(m->stream));
+    if (!report(cb, user, 1.0f)) return fail(MI_E_CANCELED, "Operation was canceled");
+    return MI_OK;
+}
+
+int mi_proj_find_projections(mi_mesh* m, const float* pts, int64_t n, const float* objXf, const float* refObjXf,
+                             float up, float lo, mi_projection* res)
+{
+    if (!m || n < 0 || (n > 0 && (!pts || !res))) return fail(MI_E_INVALID, "invalid arguments");
+    // createProjectionTransforms (source/MRMesh/MRMeshProject.cpp:105-121)
+    const float* treeXf = (refObjXf && !isRigid(refObjXf)) ? refObjXf : nullptr;
+    float folded[12];
+    const float* ptXf = nullptr;
+    if (treeXf || !refObjXf) {
+        ptXf = objXf;
+    } else {
+        xfInverse(refObjXf, folded);
+        if (objXf) xfCompose(folded, objXf, folded);
+        ptXf = folded;
+    }
+    mi::Xf px, tx;
+    if (ptXf) px = toXf(ptXf);
+    if (treeXf) tx = toXf(treeXf);
+    (void)kIdentityXf;
+    return pointsHost(m, pts, n, sizeof(mi_projection), res,
+                      [&](const float* dp, const uint32_t* perm, int64_t sz, void* dr) {
+                          return mi::launchProjections(m->dev, dp, perm, sz, ptXf ? &px : nullptr,
+                                                       treeXf ? &tx : nullptr, up, lo,
+                                                       static_cast<mi::ProjRec*>(dr), m->stackD, m->stream);
+                      },
+                      nullptr, nullptr);
+}
+
+size_t mi_proj_heap_bytes(size_t numProjections)
+{
+    return numProjections * (12 + sizeof(mi_projection) + 20);
+}
+
+} // extern "C"

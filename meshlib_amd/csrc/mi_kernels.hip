@@ -1,0 +1,574 @@
+// mi_kernels.hip -- gfx950 kernels of the distance-field path.
+//
+// Compiled with -ffp-contract=off: the box distance, the FP64 closest point, the
+// dipole test and the solid angle are evaluated with the reference CPU build's
+// rounding and operation order, so distances and projections are bit-identical to
+// the CPU path (source/MRMesh/MRMeshProject.cpp:13-103) and winding numbers differ
+// only by atan2f ulps.
+//
+// Execution model (one lane = one query, 64-lane waves):
+//  * queries of a wave are spatially coherent: grid voxels are mapped as 4x4x4
+//    bricks per wave; free points are Morton-sorted first, so the lanes of a wave
+//    walk mostly the same nodes and their loads hit the same cache lines;
+//  * each lane keeps the reference's depth-first order exactly (nearer child first,
+//    strict '<' on distances), with the current node in registers and the pending
+//    far children on a per-lane stack in LDS laid out [depth][lane] (conflict-free
+//    when lanes sit at the same depth);
+//  * blocks are remapped so that each XCD (blocks are dealt round-robin over the 8
+//    XCDs) gets one contiguous range of bricks / sorted queries, keeping each L2's
+//    working set spatially local.
+#include "mi_kernels.hpp"
+
+#include <hipcub/hipcub.hpp>
+
+#include <cfloat>
+#include <cstdint>
+
+namespace mi {
+namespace {
+
+constexpr int kBlock = 256;          // 4 waves
+constexpr int kXcds = 8;
+constexpr float kInv4Pi = 1.0f / (4 * 3.14159265358979f);   // INV_4PI, MRDipole.cpp:125
+
+__device__ __forceinline__ int logicalBlock(int nLogical)
+{
+    // blocks are dealt to XCDs round-robin; give XCD x the contiguous range
+    // [x*per, (x+1)*per) of logical blocks
+    const int per = (nLogical + kXcds - 1) / kXcds;
+    const int b = blockIdx.x;
+    return (b % kXcds) * per + b / kXcds;
+}
+
+inline int paddedGrid(int nLogical)
+{
+    const int per = (nLogical + kXcds - 1) / kXcds;
+    return per * kXcds;
+}
+
+// Box::getDistanceSq(pt) (source/MRMesh/MRBox.h:207-220): the per-dim term is the
+// distance to the nearer face outside the box, zero inside; summed x, y, z.
+__device__ __forceinline__ float boxDistSq(float mnx, float mny, float mnz, float mxx, float mxy,
+                                           float mxz, float px, float py, float pz)
+{
+    const float tx = fmaxf(fmaxf(mnx - px, px - mxx), 0.0f);
+    const float ty = fmaxf(fmaxf(mny - py, py - mxy), 0.0f);
+    const float tz = fmaxf(fmaxf(mnz - pz, pz - mxz), 0.0f);
+    return tx * tx + ty * ty + tz * tz;
+}
+
+__device__ __forceinline__ void xfApply(const Xf& xf, float& x, float& y, float& z)
+{
+    const float rx = xf.m[0] * x + xf.m[1] * y + xf.m[2] * z;
+    const float ry = xf.m[3] * x + xf.m[4] * y + xf.m[5] * z;
+    const float rz = xf.m[6] * x + xf.m[7] * y + xf.m[8] * z;
+    x = rx + xf.m[9];
+    y = ry + xf.m[10];
+    z = rz + xf.m[11];
+}
+
+// transformed(box, xf): bbox of the 8 transformed corners (source/MRMesh/MRBox.h:336-344)
+__device__ __forceinline__ void xfBox(const Xf& xf, float* mn, float* mx)
+{
+    float omn[3] = { FLT_MAX, FLT_MAX, FLT_MAX }, omx[3] = { -FLT_MAX, -FLT_MAX, -FLT_MAX };
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        float x = (c & 1) ? mx[0] : mn[0], y = (c & 2) ? mx[1] : mn[1], z = (c & 4) ? mx[2] : mn[2];
+        xfApply(xf, x, y, z);
+        omn[0] = x < omn[0] ? x : omn[0]; omx[0] = x > omx[0] ? x : omx[0];
+        omn[1] = y < omn[1] ? y : omn[1]; omx[1] = y > omx[1] ? y : omx[1];
+        omn[2] = z < omn[2] ? z : omn[2]; omx[2] = z > omx[2] ? z : omx[2];
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) { mn[i] = omn[i]; mx[i] = omx[i]; }
+}
+
+struct Leaf { float a[3], b[3], c[3]; int32_t face; };
+
+__device__ __forceinline__ Leaf loadLeaf(const float4* __restrict__ tri, int k)
+{
+    const float4 t0 = tri[3 * k], t1 = tri[3 * k + 1], t2 = tri[3 * k + 2];
+    Leaf L;
+    L.a[0] = t0.x; L.a[1] = t0.y; L.a[2] = t0.z;
+    L.b[0] = t0.w; L.b[1] = t1.x; L.b[2] = t1.y;
+    L.c[0] = t1.z; L.c[1] = t1.w; L.c[2] = t2.x;
+    L.face = __float_as_int(t2.y);
+    return L;
+}
+
+__device__ __forceinline__ double dot3(double ax, double ay, double az, double bx, double by, double bz)
+{
+    return ax * bx + ay * by + az * bz;
+}
+
+// closestPointInTriangle<double> (source/MRMesh/MRClosestPointInTriangle.h:27-87): Voronoi
+// regions #1 (a), #2 (b), #3 (c), #4 (ab), #5 (ac), #6 (bc), #0 (interior).
+__device__ __forceinline__ void closestPointD(const double p[3], const double a[3], const double b[3],
+                                              const double c[3], double pr[3], double& ba, double& bb)
+{
+    const double abx = b[0] - a[0], aby = b[1] - a[1], abz = b[2] - a[2];
+    const double acx = c[0] - a[0], acy = c[1] - a[1], acz = c[2] - a[2];
+    const double apx = p[0] - a[0], apy = p[1] - a[1], apz = p[2] - a[2];
+    const double d1 = dot3(abx, aby, abz, apx, apy, apz);
+    const double d2 = dot3(acx, acy, acz, apx, apy, apz);
+    if (d1 <= 0 && d2 <= 0) { pr[0] = a[0]; pr[1] = a[1]; pr[2] = a[2]; ba = 0; bb = 0; return; }
+    const double bpx = p[0] - b[0], bpy = p[1] - b[1], bpz = p[2] - b[2];
+    const double d3 = dot3(abx, aby, abz, bpx, bpy, bpz);
+    const double d4 = dot3(acx, acy, acz, bpx, bpy, bpz);
+    if (d3 >= 0 && d4 <= d3) { pr[0] = b[0]; pr[1] = b[1]; pr[2] = b[2]; ba = 1; bb = 0; return; }
+    const double cpx = p[0] - c[0], cpy = p[1] - c[1], cpz = p[2] - c[2];
+    const double d5 = dot3(abx, aby, abz, cpx, cpy, cpz);
+    const double d6 = dot3(acx, acy, acz, cpx, cpy, cpz);
+    if (d6 >= 0 && d5 <= d6) { pr[0] = c[0]; pr[1] = c[1]; pr[2] = c[2]; ba = 0; bb = 1; return; }
+    const double vc = d1 * d4 - d3 * d2;
+    if (vc <= 0 && d1 >= 0 && d3 <= 0) {
+        const double v = d1 / (d1 - d3);
+        pr[0] = a[0] + v * abx; pr[1] = a[1] + v * aby; pr[2] = a[2] + v * abz;
+        ba = v; bb = 0; return;
+    }
+    const double vb = d5 * d2 - d1 * d6;
+    if (vb <= 0 && d6 <= 0 && d2 >= 0) {
+        const double v = d2 / (d2 - d6);
+        pr[0] = a[0] + v * acx; pr[1] = a[1] + v * acy; pr[2] = a[2] + v * acz;
+        ba = 0; bb = v; return;
+    }
+    const double va = d3 * d6 - d5 * d4;
+    if (va <= 0) {
+        if (d4 < d3) { pr[0] = b[0]; pr[1] = b[1]; pr[2] = b[2]; ba = 1; bb = 0; return; }
+        if (d5 < d6) { pr[0] = c[0]; pr[1] = c[1]; pr[2] = c[2]; ba = 0; bb = 1; return; }
+        const double v = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+        pr[0] = b[0] + v * (c[0] - b[0]); pr[1] = b[1] + v * (c[1] - b[1]); pr[2] = b[2] + v * (c[2] - b[2]);
+        ba = 1 - v; bb = v; return;
+    }
+    const double denom = 1 / (va + vb + vc);
+    const double v = vb * denom, w = vc * denom;
+    pr[0] = a[0] + v * abx + w * acx;
+    pr[1] = a[1] + v * aby + w * acy;
+    pr[2] = a[2] + v * abz + w * acz;
+    ba = v; bb = w;
+}
+
+// triangleSolidAngle (source/MRMesh/MRDipole.cpp:81-92), det as MRMatrix3.h:226-232
+__device__ __forceinline__ float solidAngle(float px, float py, float pz, const Leaf& L)
+{
+    const float mxx = L.a[0] - px, mxy = L.a[1] - py, mxz = L.a[2] - pz;
+    const float myx = L.b[0] - px, myy = L.b[1] - py, myz = L.b[2] - pz;
+    const float mzx = L.c[0] - px, mzy = L.c[1] - py, mzz = L.c[2] - pz;
+    const float x = sqrtf(mxx * mxx + mxy * mxy + mxz * mxz);
+    const float y = sqrtf(myx * myx + myy * myy + myz * myz);
+    const float z = sqrtf(mzx * mzx + mzy * mzy + mzz * mzz);
+    const float dxy = mxx * myx + mxy * myy + mxz * myz;
+    const float dyz = myx * mzx + myy * mzy + myz * mzz;
+    const float dzx = mzx * mxx + mzy * mxy + mzz * mxz;
+    const float den = x * y * z + dxy * z + dyz * x + dzx * y;
+    const float det = mxx * (myy * mzz - myz * mzy) - mxy * (myx * mzz - myz * mzx)
+                    + mxz * (myx * mzy - myy * mzx);
+    return 2 * atan2f(det, den);
+}
+
+struct BestHit { float distSq; int32_t face; float px, py, pz, a, b; };
+
+// findProjectionSubtree (source/MRMesh/MRMeshProject.cpp:13-103) for one lane.
+// FULL: keep the winning face/point/barycentrics (projections) or only distSq (SDF).
+// XF: apply a non-rigid tree transform to boxes and triangles.
+template <bool FULL, bool XF, int D>
+__device__ __forceinline__ void distanceDfs(const MeshDev& m, float px, float py, float pz,
+                                            float up, float lo, const Xf& treeXf, int2* stk,
+                                            BestHit& res)
+{
+    res.distSq = up;
+    res.face = -1;
+    if (m.numFaces <= 0) return;
+    float rmn[3] = { m.rootMin[0], m.rootMin[1], m.rootMin[2] };
+    float rmx[3] = { m.rootMax[0], m.rootMax[1], m.rootMax[2] };
+    if (XF) xfBox(treeXf, rmn, rmx);
+    int32_t cur = m.rootRef;
+    float curd = boxDistSq(rmn[0], rmn[1], rmn[2], rmx[0], rmx[1], rmx[2], px, py, pz);
+    bool have = curd < res.distSq;
+    int sp = 0;
+    const double pd[3] = { px, py, pz };
+    for (;;) {
+        if (!have) {
+            // pop; entries whose box is no closer than the best so far are dropped
+            if (sp == 0) break;
+            --sp;
+            const int2 e = stk[sp * kBlock];
+            cur = e.x;
+            curd = __int_as_float(e.y);
+            if (curd >= res.distSq) continue;
+        }
+        have = false;
+        if (cur < 0) {
+            Leaf L = loadLeaf(m.tri, ~cur);
+            if (XF) {
+                xfApply(treeXf, L.a[0], L.a[1], L.a[2]);
+                xfApply(treeXf, L.b[0], L.b[1], L.b[2]);
+                xfApply(treeXf, L.c[0], L.c[1], L.c[2]);
+            }
+            const double ad[3] = { L.a[0], L.a[1], L.a[2] };
+            const double bd[3] = { L.b[0], L.b[1], L.b[2] };
+            const double cd[3] = { L.c[0], L.c[1], L.c[2] };
+            double pr[3], ba, bb;
+            closestPointD(pd, ad, bd, cd, pr, ba, bb);
+            const float qx = (float)pr[0], qy = (float)pr[1], qz = (float)pr[2];
+            const float dx = qx - px, dy = qy - py, dz = qz - pz;
+            const float d = dx * dx + dy * dy + dz * dz;
+            if (d < res.distSq) {
+                res.distSq = d;
+                if (FULL) {
+                    res.face = L.face;
+                    res.px = qx; res.py = qy; res.pz = qz;
+                    res.a = (float)ba; res.b = (float)bb;
+                }
+                if (d <= lo) break;
+            }
+            continue;
+        }
+        const float4 n0 = m.dnode[4 * cur], n1 = m.dnode[4 * cur + 1];
+        const float4 n2 = m.dnode[4 * cur + 2], n3 = m.dnode[4 * cur + 3];
+        float lmn[3] = { n0.x, n0.y, n0.z }, lmx[3] = { n0.w, n1.x, n1.y };
+        float rmn2[3] = { n1.z, n1.w, n2.x }, rmx2[3] = { n2.y, n2.z, n2.w };
+        if (XF) { xfBox(treeXf, lmn, lmx); xfBox(treeXf, rmn2, rmx2); }
+        const float d1 = boxDistSq(lmn[0], lmn[1], lmn[2], lmx[0], lmx[1], lmx[2], px, py, pz);
+        const float d2 = boxDistSq(rmn2[0], rmn2[1], rmn2[2], rmx2[0], rmx2[1], rmx2[2], px, py, pz);
+        const int32_t lref = __float_as_int(n3.x), rref = __float_as_int(n3.y);
+        // nearer child first; equal distances explore the right child first
+        const bool leftNear = d1 < d2;
+        const int32_t nearRef = leftNear ? lref : rref, farRef = leftNear ? rref : lref;
+        const float nearD = leftNear ? d1 : d2, farD = leftNear ? d2 : d1;
+        if (farD < res.distSq) {
+            stk[sp * kBlock] = make_int2(farRef, __float_as_int(farD));
+            ++sp;
+        }
+        if (nearD < res.distSq) { cur = nearRef; curd = nearD; have = true; }
+    }
+}
+
+// calcFastWindingNumber (source/MRMesh/MRDipole.cpp:94-128) for one lane; the stack
+// holds node refs in the .x component of the shared LDS column.
+__device__ __forceinline__ float windingNumber(const MeshDev& m, float qx, float qy, float qz,
+                                               float beta, int32_t skipFace, int2* stk)
+{
+    if (m.numFaces <= 0) return 0;
+    const float betaSq = beta * beta;
+    float res = 0;
+    int sp = 0;
+    int32_t cur = m.rootRef;
+    for (;;) {
+        float4 d0, d1;
+        if (cur >= 0) { d0 = m.dint[2 * cur]; d1 = m.dint[2 * cur + 1]; }
+        else { d0 = m.dleaf[2 * (~cur)]; d1 = m.dleaf[2 * (~cur) + 1]; }
+        // Dipole::addIfGoodApprox (source/MRMesh/MRDipole.h:18-27)
+        const float dpx = d0.x - qx, dpy = d0.y - qy, dpz = d0.z - qz;
+        const float dd = dpx * dpx + dpy * dpy + dpz * dpz;
+        if (!(dd <= betaSq * d0.w)) {
+            const float dist = sqrtf(dd);
+            if (dist > 0) res += (dpx * d1.x + dpy * d1.y + dpz * d1.z) / (dist * dd);
+        } else if (cur >= 0) {
+            const float4 n3 = m.dnode[4 * cur + 3];
+            stk[sp * kBlock].x = __float_as_int(n3.y);   // right child later
+            ++sp;
+            cur = __float_as_int(n3.x);                   // left child next
+            continue;
+        } else {
+            const Leaf L = loadLeaf(m.tri, ~cur);
+            if (L.face != skipFace) res += solidAngle(qx, qy, qz, L);
+        }
+        if (sp == 0) break;
+        --sp;
+        cur = stk[sp * kBlock].x;
+    }
+    return kInv4Pi * res;
+}
+
+// voxel of lane `lane` in brick `brick` (4x4x4 per wave, bricks x-fastest)
+__device__ __forceinline__ bool brickVoxel(int brick, int lane, int nbx, int nby, int nbz, int dx,
+                                           int dy, int zBegin, int zEnd, int& x, int& y, int& z)
+{
+    const int bx = brick % nbx;
+    const int t = brick / nbx;
+    const int by = t % nby;
+    const int bz = t / nby;
+    x = bx * 4 + (lane & 3);
+    y = by * 4 + ((lane >> 2) & 3);
+    z = zBegin + bz * 4 + (lane >> 4);
+    return bz < nbz && x < dx && y < dy && z < zEnd;
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void kSdfGrid(MeshDev m, int dx, int dy, int zBegin, int zEnd,
+                                                   int nbx, int nby, int nbz, int nLogical, Xf xf,
+                                                   DistOpts o, float* __restrict__ out)
+{
+    __shared__ int2 smem[D * kBlock];
+    const int lb = logicalBlock(nLogical);
+    if (lb >= nLogical) return;
+    const int lane = threadIdx.x & 63;
+    const int brick = lb * (kBlock / 64) + (threadIdx.x >> 6);
+    int x, y, z;
+    if (!brickVoxel(brick, lane, nbx, nby, nbz, dx, dy, zBegin, zEnd, x, y, z)) return;
+    float px = (float)x, py = (float)y, pz = (float)z;
+    xfApply(xf, px, py, pz);
+    int2* stk = smem + threadIdx.x;
+    BestHit h;
+    distanceDfs<false, false, D>(m, px, py, pz, o.maxDistSq, o.minDistSq, xf, stk, h);
+    const size_t idx = ((size_t)(z - zBegin) * dy + y) * dx + x;
+    const float resSq = h.distSq;
+    // FastWindingNumber::calcWithDistances (source/MRMesh/MRFastWindingNumber.cpp:69-76)
+    if (o.nullOutsideMinMax && (resSq < o.minDistSq || resSq >= o.maxDistSq)) {
+        out[idx] = __int_as_float(0x7fc00000);
+        return;
+    }
+    const float wn = windingNumber(m, px, py, pz, o.beta, -1, stk);
+    const float sign = wn > o.threshold ? -1.f : +1.f;
+    out[idx] = sign * sqrtf(resSq);
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void kFwnGrid(MeshDev m, int dx, int dy, int zBegin, int zEnd,
+                                                   int nbx, int nby, int nbz, int nLogical, Xf xf,
+                                                   float beta, float* __restrict__ out)
+{
+    __shared__ int2 smem[D * kBlock];
+    const int lb = logicalBlock(nLogical);
+    if (lb >= nLogical) return;
+    const int lane = threadIdx.x & 63;
+    const int brick = lb * (kBlock / 64) + (threadIdx.x >> 6);
+    int x, y, z;
+    if (!brickVoxel(brick, lane, nbx, nby, nbz, dx, dy, zBegin, zEnd, x, y, z)) return;
+    float px = (float)x, py = (float)y, pz = (float)z;
+    xfApply(xf, px, py, pz);
+    const size_t idx = ((size_t)(z - zBegin) * dy + y) * dx + x;
+    out[idx] = windingNumber(m, px, py, pz, beta, -1, smem + threadIdx.x);
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void kFwnPoints(MeshDev m, const float* __restrict__ pts,
+                                                     const uint32_t* __restrict__ perm, int64_t n,
+                                                     int nLogical, float beta, int32_t skipFace,
+                                                     float* __restrict__ out)
+{
+    __shared__ int2 smem[D * kBlock];
+    const int lb = logicalBlock(nLogical);
+    if (lb >= nLogical) return;
+    const int64_t i = (int64_t)lb * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int64_t q = perm ? (int64_t)perm[i] : i;
+    const float px = pts[3 * q], py = pts[3 * q + 1], pz = pts[3 * q + 2];
+    out[q] = windingNumber(m, px, py, pz, beta, skipFace, smem + threadIdx.x);
+}
+
+// FastWindingNumber::calcSelfIntersections (source/MRMesh/MRFastWindingNumber.cpp:40-52): one
+// lane per leaf slot (tree order = spatial order), query = triCenter, own face skipped
+template <int D>
+__global__ __launch_bounds__(kBlock) void kSelfIntersections(MeshDev m, int nLogical, float beta,
+                                                             uint8_t* __restrict__ out)
+{
+    __shared__ int2 smem[D * kBlock];
+    const int lb = logicalBlock(nLogical);
+    if (lb >= nLogical) return;
+    const int k = lb * kBlock + threadIdx.x;
+    if (k >= m.numFaces) return;
+    const Leaf L = loadLeaf(m.tri, k);
+    const float third = 1 / 3.0f;
+    const float qx = third * ((L.a[0] + L.b[0]) + L.c[0]);
+    const float qy = third * ((L.a[1] + L.b[1]) + L.c[1]);
+    const float qz = third * ((L.a[2] + L.b[2]) + L.c[2]);
+    const float wn = windingNumber(m, qx, qy, qz, beta, L.face, smem + threadIdx.x);
+    out[L.face] = (wn < 0 || wn > 1) ? 1 : 0;
+}
+
+template <bool XF, int D>
+__global__ __launch_bounds__(kBlock) void kProjections(MeshDev m, const float* __restrict__ pts,
+                                                       const uint32_t* __restrict__ perm, int64_t n,
+                                                       int nLogical, Xf ptXf, int hasPtXf, Xf treeXf,
+                                                       float up, float lo, ProjRec* __restrict__ out)
+{
+    __shared__ int2 smem[D * kBlock];
+    const int lb = logicalBlock(nLogical);
+    if (lb >= nLogical) return;
+    const int64_t i = (int64_t)lb * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int64_t q = perm ? (int64_t)perm[i] : i;
+    float px = pts[3 * q], py = pts[3 * q + 1], pz = pts[3 * q + 2];
+    if (hasPtXf) xfApply(ptXf, px, py, pz);
+    BestHit h;
+    h.px = h.py = h.pz = 0; h.a = h.b = 0;
+    distanceDfs<true, XF, D>(m, px, py, pz, up, lo, treeXf, smem + threadIdx.x, h);
+    float4* o = reinterpret_cast<float4*>(out + q);
+    o[0] = make_float4(__int_as_float(h.face), h.px, h.py, h.pz);
+    o[1] = make_float4(__int_as_float(-1), h.a, h.b, h.distSq);
+}
+
+__device__ __forceinline__ uint32_t spread10(uint32_t v)
+{
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000ffu;
+    v = (v | (v << 8)) & 0x0300f00fu;
+    v = (v | (v << 4)) & 0x030c30c3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+__global__ void kMorton(const float* __restrict__ pts, int64_t n, float lx, float ly, float lz,
+                        float sx, float sy, float sz, uint32_t* __restrict__ keys,
+                        uint32_t* __restrict__ vals)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float fx = fminf(fmaxf((pts[3 * i] - lx) * sx, 0.f), 1023.f);
+    const float fy = fminf(fmaxf((pts[3 * i + 1] - ly) * sy, 0.f), 1023.f);
+    const float fz = fminf(fmaxf((pts[3 * i + 2] - lz) * sz, 0.f), 1023.f);
+    keys[i] = spread10((uint32_t)fx) | (spread10((uint32_t)fy) << 1) | (spread10((uint32_t)fz) << 2);
+    vals[i] = (uint32_t)i;
+}
+
+template <typename Fn>
+hipError_t dispatchDepth(int D, Fn&& fn)
+{
+    switch (D) {
+    case 8: return fn(std::integral_constant<int, 8>{});
+    case 16: return fn(std::integral_constant<int, 16>{});
+    case 24: return fn(std::integral_constant<int, 24>{});
+    case 32: return fn(std::integral_constant<int, 32>{});
+    default: return hipErrorInvalidValue;
+    }
+}
+
+} // namespace
+
+int stackDepthFor(int treeDepth)
+{
+    // pending far children of the ancestors of the current node: at most treeDepth entries
+    if (treeDepth <= 8) return 8;
+    if (treeDepth <= 16) return 16;
+    if (treeDepth <= 24) return 24;
+    if (treeDepth <= 32) return 32;
+    return 0;
+}
+
+hipError_t launchSdfGrid(const MeshDev& m, const int32_t dims[3], const Xf& xf, const DistOpts& o,
+                         int32_t zBegin, int32_t zEnd, float* out, int stackD, hipStream_t s)
+{
+    const int nbx = (dims[0] + 3) / 4, nby = (dims[1] + 3) / 4, nbz = (zEnd - zBegin + 3) / 4;
+    const int64_t bricks = (int64_t)nbx * nby * nbz;
+    if (bricks <= 0) return hipSuccess;
+    const int nLogical = (int)((bricks + 3) / 4);
+    return dispatchDepth(stackD, [&](auto Dc) {
+        constexpr int D = decltype(Dc)::value;
+        hipLaunchKernelGGL(kSdfGrid<D>, dim3(paddedGrid(nLogical)), dim3(kBlock), 0, s, m, dims[0],
+                           dims[1], zBegin, zEnd, nbx, nby, nbz, nLogical, xf, o, out);
+        return hipGetLastError();
+    });
+}
+
+hipError_t launchFwnGrid(const MeshDev& m, const int32_t dims[3], const Xf& xf, float beta,
+                         int32_t zBegin, int32_t zEnd, float* out, int stackD, hipStream_t s)
+{
+    const int nbx = (dims[0] + 3) / 4, nby = (dims[1] + 3) / 4, nbz = (zEnd - zBegin + 3) / 4;
+    const int64_t bricks = (int64_t)nbx * nby * nbz;
+    if (bricks <= 0) return hipSuccess;
+    const int nLogical = (int)((bricks + 3) / 4);
+    return dispatchDepth(stackD, [&](auto Dc) {
+        constexpr int D = decltype(Dc)::value;
+        hipLaunchKernelGGL(kFwnGrid<D>, dim3(paddedGrid(nLogical)), dim3(kBlock), 0, s, m, dims[0],
+                           dims[1], zBegin, zEnd, nbx, nby, nbz, nLogical, xf, beta, out);
+        return hipGetLastError();
+    });
+}
+
+hipError_t launchFwnPoints(const MeshDev& m, const float* pts, const uint32_t* perm, int64_t n,
+                           float beta, int32_t skipFace, float* out, int stackD, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    const int nLogical = (int)((n + kBlock - 1) / kBlock);
+    return dispatchDepth(stackD, [&](auto Dc) {
+        constexpr int D = decltype(Dc)::value;
+        hipLaunchKernelGGL(kFwnPoints<D>, dim3(paddedGrid(nLogical)), dim3(kBlock), 0, s, m, pts,
+                           perm, n, nLogical, beta, skipFace, out);
+        return hipGetLastError();
+    });
+}
+
+hipError_t launchSelfIntersections(const MeshDev& m, float beta, uint8_t* out, int stackD, hipStream_t s)
+{
+    if (m.numFaces <= 0) return hipSuccess;
+    const int nLogical = (m.numFaces + kBlock - 1) / kBlock;
+    return dispatchDepth(stackD, [&](auto Dc) {
+        constexpr int D = decltype(Dc)::value;
+        hipLaunchKernelGGL(kSelfIntersections<D>, dim3(paddedGrid(nLogical)), dim3(kBlock), 0, s, m,
+                           nLogical, beta, out);
+        return hipGetLastError();
+    });
+}
+
+hipError_t launchProjections(const MeshDev& m, const float* pts, const uint32_t* perm, int64_t n,
+                             const Xf* ptXf, const Xf* treeXf, float up, float lo, ProjRec* out,
+                             int stackD, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    const int nLogical = (int)((n + kBlock - 1) / kBlock);
+    Xf px{}, tx{};
+    if (ptXf) px = *ptXf;
+    if (treeXf) tx = *treeXf;
+    return dispatchDepth(stackD, [&](auto Dc) {
+        constexpr int D = decltype(Dc)::value;
+        if (treeXf)
+            hipLaunchKernelGGL((kProjections<true, D>), dim3(paddedGrid(nLogical)), dim3(kBlock), 0, s,
+                               m, pts, perm, n, nLogical, px, ptXf ? 1 : 0, tx, up, lo, out);
+        else
+            hipLaunchKernelGGL((kProjections<false, D>), dim3(paddedGrid(nLogical)), dim3(kBlock), 0, s,
+                               m, pts, perm, n, nLogical, px, ptXf ? 1 : 0, tx, up, lo, out);
+        return hipGetLastError();
+    });
+}
+
+namespace {
+inline size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
+size_t cubTempBytes(int64_t n)
+{
+    size_t bytes = 0;
+    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                       (const uint32_t*)nullptr, (uint32_t*)nullptr, (int)n, 0, 30);
+    return bytes;
+}
+} // namespace
+
+size_t sortWorkspaceBytes(int64_t n)
+{
+    if (n <= 0) return 0;
+    return 4 * align256((size_t)n * 4) + align256(cubTempBytes(n));
+}
+
+hipError_t sortPointsMorton(const float* pts, int64_t n, const float lo[3], const float hi[3],
+                            void* ws, size_t wsBytes, const uint32_t** permOut, hipStream_t s)
+{
+    *permOut = nullptr;
+    if (n <= 0) return hipSuccess;
+    if (n > INT32_MAX || wsBytes < sortWorkspaceBytes(n)) return hipErrorInvalidValue;
+    char* p = static_cast<char*>(ws);
+    const size_t a = align256((size_t)n * 4);
+    uint32_t* keysIn = reinterpret_cast<uint32_t*>(p);
+    uint32_t* keysOut = reinterpret_cast<uint32_t*>(p + a);
+    uint32_t* valsIn = reinterpret_cast<uint32_t*>(p + 2 * a);
+    uint32_t* valsOut = reinterpret_cast<uint32_t*>(p + 3 * a);
+    void* temp = p + 4 * a;
+    size_t tempBytes = cubTempBytes(n);
+    float sc[3];
+    for (int i = 0; i < 3; ++i) {
+        const float ext = hi[i] - lo[i];
+        sc[i] = ext > 0 ? 1023.f / ext : 0.f;
+    }
+    const int tb = 256;
+    hipLaunchKernelGGL(kMorton, dim3((unsigned)((n + tb - 1) / tb)), dim3(tb), 0, s, pts, n, lo[0], lo[1],
+                       lo[2], sc[0], sc[1], sc[2], keysIn, valsIn);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    e = hipcub::DeviceRadixSort::SortPairs(temp, tempBytes, keysIn, keysOut, valsIn, valsOut, (int)n, 0,
+                                           30, s);
+    if (e != hipSuccess) return e;
+    *permOut = valsOut;
+    return hipSuccess;
+}
+
+} // namespace mi

@@ -1,0 +1,53 @@
+// mi_kernels.hpp -- launchers of the gfx950 kernels (internal to the library).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+namespace mi {
+
+// device view of one mesh's flattened tree (layout in mi_tree.hpp)
+struct MeshDev {
+    const float4* dnode;   // 4 per internal node
+    const float4* tri;     // 3 per leaf slot
+    const float4* dint;    // 2 per internal node
+    const float4* dleaf;   // 2 per leaf slot
+    float rootMin[3], rootMax[3];
+    int32_t rootRef;
+    int32_t numFaces;
+};
+
+struct Xf { float m[12]; };   // row-major A then b (AffineXf3f)
+
+struct DistOpts { float minDistSq, maxDistSq; int32_t nullOutsideMinMax; float threshold, beta; };
+
+struct ProjRec { int32_t face; float px, py, pz; int32_t e; float a, b; float distSq; };
+static_assert(sizeof(ProjRec) == 32, "MeshProjectionResult is 32 B");
+
+// kernel variant knobs chosen by the autotuner
+struct LaunchCfg { int blockSize = 256; int xcdRemap = 1; };
+
+// stack depth template actually used for a tree of the given depth (0 if unsupported)
+int stackDepthFor(int treeDepth);
+
+// z-layer slab [zBegin, zEnd) of the grid, output x-fastest
+hipError_t launchSdfGrid(const MeshDev& m, const int32_t dims[3], const Xf& xf, const DistOpts& o,
+                         int32_t zBegin, int32_t zEnd, float* out, int stackD, hipStream_t s);
+hipError_t launchFwnGrid(const MeshDev& m, const int32_t dims[3], const Xf& xf, float beta,
+                         int32_t zBegin, int32_t zEnd, float* out, int stackD, hipStream_t s);
+// perm may be null (identity order)
+hipError_t launchFwnPoints(const MeshDev& m, const float* pts, const uint32_t* perm, int64_t n,
+                           float beta, int32_t skipFace, float* out, int stackD, hipStream_t s);
+hipError_t launchSelfIntersections(const MeshDev& m, float beta, uint8_t* out, int stackD, hipStream_t s);
+hipError_t launchProjections(const MeshDev& m, const float* pts, const uint32_t* perm, int64_t n,
+                             const Xf* ptXf, const Xf* treeXf, float up, float lo, ProjRec* out,
+                             int stackD, hipStream_t s);
+
+// spatial ordering of query points: Morton codes over box [lo, hi] then a radix sort.
+// workspace layout is private; query its size first with ws == nullptr.
+size_t sortWorkspaceBytes(int64_t n);
+hipError_t sortPointsMorton(const float* pts, int64_t n, const float lo[3], const float hi[3],
+                            void* ws, size_t wsBytes, const uint32_t** permOut, hipStream_t s);
+
+} // namespace mi
