@@ -38,6 +38,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--traversal", default="wave_coop", choices=["wave_coop", "per_lane"],
+                    help="kernel traversal form of the measured (value) run")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="skip the one-step comparison run with the per-lane (reference-form) traversal")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: this build has no multi-GPU path (asked for --gpus "
@@ -83,6 +87,7 @@ def main():
         if timed:
             return True
 
+    M.set_traversal(args.traversal)
     for i in range(args.warmup):
         step(False)
         torch.cuda.synchronize()
@@ -100,6 +105,25 @@ def main():
     ms_per_step = 1000.0 * t_total / args.steps
     sdf_avg = float(np.mean(sdf_ms))
     proj_avg = float(np.mean(proj_ms))
+    res_sdf = d_sdf.cpu()
+    res_proj = d_proj.cpu()
+
+    # one step with the per-lane traversal (the reference's kernel form) for the speedup figure;
+    # outputs must be identical
+    plain = None
+    if not args.no_plain and args.traversal != "per_lane":
+        M.set_traversal("per_lane")
+        step(True)
+        torch.cuda.synchronize()
+        p_sdf, p_proj = ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2])
+        same_sdf = bool(torch.equal(d_sdf.cpu().view(torch.int32), res_sdf.view(torch.int32)))
+        same_proj = bool(torch.equal(d_proj.cpu(), res_proj))
+        M.set_traversal(args.traversal)
+        plain = {"traversal": "per_lane", "sdf_ms": p_sdf, "proj_ms": p_proj,
+                 "sdf_speedup": p_sdf / sdf_avg, "proj_speedup": p_proj / proj_avg,
+                 "step_speedup": (p_sdf + p_proj) / (sdf_avg + proj_avg),
+                 "identical_sdf": same_sdf, "identical_projections": same_proj}
+        log(f"per-lane: sdf {p_sdf:.2f} ms, projections {p_proj:.2f} ms, identical {same_sdf} {same_proj}")
     vox_rate = nvox / (sdf_avg / 1000.0)
     pts_rate = NQ / (proj_avg / 1000.0)
     value = (nvox + NQ) / (ms_per_step / 1000.0)
@@ -147,7 +171,8 @@ def main():
     roof = None
     if b_vox is not None:
         achieved = b_vox * nvox / (sdf_avg / 1000.0) / 1e9
-        roof = {"bound": "hbm", "kernel": "kSdfGrid", "achieved": achieved, "peak": HBM_PEAK_GBS,
+        kname = "kSdfGridCoop" if args.traversal == "wave_coop" else "kSdfGrid"
+        roof = {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                 "bytes_per_voxel": b_vox, "bytes_per_point": b_pt}
     out = {
@@ -160,6 +185,7 @@ def main():
                                "grid + 2^22 point projections per step", "voxels": nvox,
                    "points": NQ, "beta": 2.0},
         "voxels_per_s": vox_rate, "pts_per_s": pts_rate, "sdf_ms": sdf_avg, "proj_ms": proj_avg,
+        "traversal": args.traversal, "vs_per_lane": plain,
         "learned_vs_plain": None,
         "roofline": roof, "cpu_baseline": cpu,
     }

@@ -85,6 +85,15 @@ size_t mi_last_error(char* buf, size_t len);
  * entry points at this many bytes (used by tests to force multi-chunk execution) */
 int mi_set_memory_limit(size_t bytes);
 
+/* Traversal form of the kernels, process-wide (no counterpart in the reference; its GPU path
+ * has one form, MRCudaFastWindingNumber.cu:31-133).  Results do not depend on it.
+ *   MI_TRAVERSAL_PER_LANE  one depth-first walk per query, the reference's form
+ *   MI_TRAVERSAL_WAVE_COOP one walk per 64-query wave with uniform node loads (default) */
+#define MI_TRAVERSAL_PER_LANE 0
+#define MI_TRAVERSAL_WAVE_COOP 1
+int mi_set_traversal(int mode);
+int mi_get_traversal(void);
+
 /* ---------------------------------------------------------------- mesh data */
 
 /* Cuda::FastWindingNumber::prepareData_ (source/MRCuda/MRCudaFastWindingNumber.cpp:46-85)

@@ -61,6 +61,23 @@ def lib_path() -> str:
     return _LIB_PATH
 
 
+def _share_hip_runtime_with_torch():
+    """One HIP runtime per process.  PyTorch-ROCm wheels bundle their own libamdhip64 (SONAME
+    libamdhip64.so.7, loaded by file name from torch/lib); this library links libamdhip64.so.7
+    from /opt/rocm.  Two copies in one process each initialise the HSA runtime and the second
+    sees no GPU.  Loading torch's copy first (RTLD_GLOBAL) makes both resolve to it: this
+    library's NEEDED entry matches the SONAME, and torch's own lookup finds the same file."""
+    import importlib.util
+    spec = importlib.util.find_spec("torch")
+    if spec is None or not spec.submodule_search_locations:
+        return
+    for base in spec.submodule_search_locations:
+        cand = os.path.join(base, "lib", "libamdhip64.so")
+        if os.path.exists(cand):
+            C.CDLL(cand, mode=C.RTLD_GLOBAL)
+            return
+
+
 def lib():
     """Loads the native library; raises if it has not been built (no fallback)."""
     global _lib
@@ -69,11 +86,14 @@ def lib():
     if not os.path.exists(_LIB_PATH):
         raise MeshLibAmdError(MI_E_INVALID, f"native library missing: {_LIB_PATH} "
                               "(build it with `python -m meshlib_amd.build`)")
+    _share_hip_runtime_with_torch()
     L = C.CDLL(_LIB_PATH)
     P, I32, I64, F, SZ = C.c_void_p, C.c_int32, C.c_int64, C.c_float, C.c_size_t
     sig = {
         "mi_last_error": (SZ, [C.c_char_p, SZ]),
         "mi_set_memory_limit": (C.c_int, [SZ]),
+        "mi_set_traversal": (C.c_int, [C.c_int]),
+        "mi_get_traversal": (C.c_int, []),
         "mi_mesh_create": (C.c_int, [P, I32, P, I32, C.POINTER(P)]),
         "mi_mesh_destroy": (None, [P]),
         "mi_mesh_num_nodes": (C.c_int, [P]),
@@ -140,6 +160,20 @@ def build_tree_host(verts, tris):
     _check(lib().mi_build_tree_host(_p(v), v.shape[0], _p(t), t.shape[0], _p(nodes), _p(dips),
                                     C.byref(depth)))
     return nodes, dips, int(depth.value)
+
+
+TRAVERSALS = {"per_lane": 0, "wave_coop": 1}
+
+
+def set_traversal(mode: str):
+    """Kernel traversal form: 'per_lane' (one walk per query, the reference's form) or
+    'wave_coop' (one walk per 64-query wave; default).  Results are identical."""
+    _check(lib().mi_set_traversal(TRAVERSALS[mode]))
+
+
+def get_traversal() -> str:
+    v = int(lib().mi_get_traversal())
+    return {b: a for a, b in TRAVERSALS.items()}[v]
 
 
 def set_memory_limit(nbytes: int):

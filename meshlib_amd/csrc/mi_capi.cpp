@@ -171,7 +171,8 @@ int ensureWorkspace(mi_mesh* m, size_t bytes)
     return MI_OK;
 }
 
-hipStream_t pick(mi_mesh* m, void* s) { return s ? static_cast<hipStream_t>(s) : m->stream; }
+// device-resident entry points run on the caller's stream; NULL is the HIP null (default) stream
+hipStream_t pick(mi_mesh*, void* s) { return static_cast<hipStream_t>(s); }
 
 int checkGridArgs(const int32_t* dims, const float* xf)
 {
@@ -200,6 +201,16 @@ int mi_set_memory_limit(size_t bytes)
     g_memLimit.store(bytes);
     return MI_OK;
 }
+
+int mi_set_traversal(int mode)
+{
+    if (mode != MI_TRAVERSAL_PER_LANE && mode != MI_TRAVERSAL_WAVE_COOP)
+        return fail(MI_E_INVALID, "unknown traversal mode");
+    mi::setTraversal(static_cast<mi::Traversal>(mode));
+    return MI_OK;
+}
+
+int mi_get_traversal(void) { return static_cast<int>(mi::traversal()); }
 
 int mi_mesh_create(const float* verts, int32_t nv, const int32_t* tris, int32_t nf, mi_mesh** out)
 {

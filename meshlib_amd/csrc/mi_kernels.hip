@@ -21,6 +21,7 @@
 
 #include <hipcub/hipcub.hpp>
 
+#include <atomic>
 #include <cfloat>
 #include <cstdint>
 
@@ -400,6 +401,235 @@ __global__ __launch_bounds__(kBlock) void kProjections(MeshDev m, const float* _
     o[1] = make_float4(__int_as_float(-1), h.a, h.b, h.distSq);
 }
 
+// ---------------------------------------------------------------- wave-cooperative traversals
+//
+// One wave walks ONE depth-first traversal for its 64 coherent queries: the node ref is
+// wave-uniform (readfirstlane), so every node record is fetched once per wave at a uniform
+// address instead of up to 64 divergent ones, and the stack is one short wave-private array of
+// uniform entries in LDS (a few hundred bytes per wave instead of D*8 bytes per lane).
+
+constexpr int kWaves = kBlock / 64;
+
+__device__ __forceinline__ uint64_t waveBallot(bool p) { return __ballot(p); }
+
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ float laneValue(float v, int lane)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+// Minimum squared distance of each lane's point to the mesh below `up` (every leaf whose box is
+// closer than the lane's current best is tested, as in findProjectionSubtree).  The result is a
+// minimum over candidates, so it does not depend on the visit order; lanes pass up < 0 to opt out.
+template <int D>
+__device__ __forceinline__ float coopDistance(const MeshDev& m, float px, float py, float pz, float up,
+                                              float4* stk)
+{
+    float best = up;
+    if (m.numFaces <= 0) return best;
+    const double pd[3] = { px, py, pz };
+    int sp = 0;
+    int cur = m.rootRef;
+    float curd = boxDistSq(m.rootMin[0], m.rootMin[1], m.rootMin[2], m.rootMax[0], m.rootMax[1],
+                           m.rootMax[2], px, py, pz);
+    bool have = true;
+    for (;;) {
+        if (!have) {
+            if (sp == 0) break;
+            --sp;
+            const float4 e0 = stk[2 * sp], e1 = stk[2 * sp + 1];
+            cur = __float_as_int(e0.x);
+            curd = boxDistSq(e0.y, e0.z, e0.w, e1.x, e1.y, e1.z, px, py, pz);
+        }
+        have = false;
+        const bool act = curd < best;
+        if (!waveBallot(act)) continue;
+        cur = uniform(cur);
+        if (cur < 0) {
+            const Leaf L = loadLeaf(m.tri, ~cur);
+            if (act) {
+                const double ad[3] = { L.a[0], L.a[1], L.a[2] };
+                const double bd[3] = { L.b[0], L.b[1], L.b[2] };
+                const double cd[3] = { L.c[0], L.c[1], L.c[2] };
+                double pr[3], ba, bb;
+                closestPointD(pd, ad, bd, cd, pr, ba, bb);
+                const float dx = (float)pr[0] - px, dy = (float)pr[1] - py, dz = (float)pr[2] - pz;
+                const float d = dx * dx + dy * dy + dz * dz;
+                if (d < best) best = d;
+            }
+            continue;
+        }
+        const float4 n0 = m.dnode[4 * cur], n1 = m.dnode[4 * cur + 1];
+        const float4 n2 = m.dnode[4 * cur + 2], n3 = m.dnode[4 * cur + 3];
+        const float d1 = boxDistSq(n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, px, py, pz);
+        const float d2 = boxDistSq(n1.z, n1.w, n2.x, n2.y, n2.z, n2.w, px, py, pz);
+        const uint64_t b1 = waveBallot(act && d1 < best), b2 = waveBallot(act && d2 < best);
+        if (!(b1 | b2)) continue;
+        bool leftFirst = b1 != 0;
+        if (b1 && b2) {
+            const int lead = __builtin_ctzll(b1 | b2);
+            leftFirst = laneValue(d1, lead) < laneValue(d2, lead);
+        }
+        // the later child goes on the stack, the earlier one stays current
+        if (leftFirst) {
+            if (b2) {
+                stk[2 * sp] = make_float4(n3.y, n1.z, n1.w, n2.x);
+                stk[2 * sp + 1] = make_float4(n2.y, n2.z, n2.w, 0.f);
+                ++sp;
+            }
+            cur = __float_as_int(n3.x); curd = d1; have = true;
+        } else {
+            if (b1) {
+                stk[2 * sp] = make_float4(n3.x, n0.x, n0.y, n0.z);
+                stk[2 * sp + 1] = make_float4(n0.w, n1.x, n1.y, 0.f);
+                ++sp;
+            }
+            cur = __float_as_int(n3.y); curd = d2; have = true;
+        }
+    }
+    return best;
+}
+
+// Winding number of each lane's point: the traversal order of calcFastWindingNumber is left-first
+// for every query, so a wave walks one left-first traversal and each entry carries the mask of
+// lanes that still descend there.  Every lane visits exactly its own node sequence in its own
+// order, so the sums are bit-identical to the per-lane traversal.  valid = false opts a lane out.
+__device__ __forceinline__ float coopWinding(const MeshDev& m, float qx, float qy, float qz, bool valid,
+                                             float beta, int32_t skipFace, uint4* stk)
+{
+    if (m.numFaces <= 0) return 0;
+    const float betaSq = beta * beta;
+    const int lane = threadIdx.x & 63;
+    float res = 0;
+    uint64_t curMask = waveBallot(valid);
+    int cur = m.rootRef;
+    int sp = 0;
+    while (curMask) {
+        const bool mine = (curMask >> lane) & 1;
+        cur = uniform(cur);
+        float4 d0, d1;
+        if (cur >= 0) { d0 = m.dint[2 * cur]; d1 = m.dint[2 * cur + 1]; }
+        else { d0 = m.dleaf[2 * (~cur)]; d1 = m.dleaf[2 * (~cur) + 1]; }
+        const float dpx = d0.x - qx, dpy = d0.y - qy, dpz = d0.z - qz;
+        const float dd = dpx * dpx + dpy * dpy + dpz * dpz;
+        const bool approx = !(dd <= betaSq * d0.w);
+        if (mine && approx) {
+            const float dist = sqrtf(dd);
+            if (dist > 0) res += (dpx * d1.x + dpy * d1.y + dpz * d1.z) / (dist * dd);
+        }
+        const uint64_t desc = waveBallot(mine && !approx);
+        if (desc) {
+            if (cur >= 0) {
+                const float4 n3 = m.dnode[4 * cur + 3];
+                stk[sp] = make_uint4(__float_as_uint(n3.y), (uint32_t)desc, (uint32_t)(desc >> 32), 0u);
+                ++sp;
+                cur = __float_as_int(n3.x);
+                curMask = desc;
+                continue;
+            }
+            const Leaf L = loadLeaf(m.tri, ~cur);
+            if (mine && !approx && L.face != skipFace) res += solidAngle(qx, qy, qz, L);
+        }
+        if (sp == 0) break;
+        --sp;
+        const uint4 e = stk[sp];
+        cur = (int)e.x;
+        curMask = (uint64_t)e.y | ((uint64_t)e.z << 32);
+    }
+    return kInv4Pi * res;
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void kSdfGridCoop(MeshDev m, int dx, int dy, int zBegin, int zEnd,
+                                                       int nbx, int nby, int nbz, int nLogical, Xf xf,
+                                                       DistOpts o, float* __restrict__ out)
+{
+    __shared__ float4 stkMem[kWaves][2 * D];
+    const int lb = logicalBlock(nLogical);
+    if (lb >= nLogical) return;
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int brick = lb * kWaves + w;
+    int x, y, z;
+    const bool valid = brickVoxel(brick, lane, nbx, nby, nbz, dx, dy, zBegin, zEnd, x, y, z);
+    if (!waveBallot(valid)) return;
+    float px = (float)x, py = (float)y, pz = (float)z;
+    xfApply(xf, px, py, pz);
+    const float resSq = coopDistance<D>(m, px, py, pz, valid ? o.maxDistSq : -1.f, stkMem[w]);
+    // FastWindingNumber::calcWithDistances (source/MRMesh/MRFastWindingNumber.cpp:69-76)
+    const bool nan = o.nullOutsideMinMax && (resSq < o.minDistSq || resSq >= o.maxDistSq);
+    const float wn = coopWinding(m, px, py, pz, valid && !nan, o.beta, -1,
+                                 reinterpret_cast<uint4*>(stkMem[w]));
+    if (!valid) return;
+    const size_t idx = ((size_t)(z - zBegin) * dy + y) * dx + x;
+    if (nan) {
+        out[idx] = __int_as_float(0x7fc00000);
+        return;
+    }
+    const float sign = wn > o.threshold ? -1.f : +1.f;
+    out[idx] = sign * sqrtf(resSq);
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void kFwnGridCoop(MeshDev m, int dx, int dy, int zBegin, int zEnd,
+                                                       int nbx, int nby, int nbz, int nLogical, Xf xf,
+                                                       float beta, float* __restrict__ out)
+{
+    __shared__ uint4 stkMem[kWaves][D];
+    const int lb = logicalBlock(nLogical);
+    if (lb >= nLogical) return;
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int brick = lb * kWaves + w;
+    int x, y, z;
+    const bool valid = brickVoxel(brick, lane, nbx, nby, nbz, dx, dy, zBegin, zEnd, x, y, z);
+    if (!waveBallot(valid)) return;
+    float px = (float)x, py = (float)y, pz = (float)z;
+    xfApply(xf, px, py, pz);
+    const float wn = coopWinding(m, px, py, pz, valid, beta, -1, stkMem[w]);
+    if (!valid) return;
+    out[((size_t)(z - zBegin) * dy + y) * dx + x] = wn;
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void kFwnPointsCoop(MeshDev m, const float* __restrict__ pts,
+                                                         const uint32_t* __restrict__ perm, int64_t n,
+                                                         int nLogical, float beta, int32_t skipFace,
+                                                         float* __restrict__ out)
+{
+    __shared__ uint4 stkMem[kWaves][D];
+    const int lb = logicalBlock(nLogical);
+    if (lb >= nLogical) return;
+    const int64_t i = (int64_t)lb * kBlock + threadIdx.x;
+    const bool valid = i < n;
+    if (!waveBallot(valid)) return;
+    const int64_t q = valid ? (perm ? (int64_t)perm[i] : i) : 0;
+    float px = 0, py = 0, pz = 0;
+    if (valid) { px = pts[3 * q]; py = pts[3 * q + 1]; pz = pts[3 * q + 2]; }
+    const float wn = coopWinding(m, px, py, pz, valid, beta, skipFace, stkMem[threadIdx.x >> 6]);
+    if (valid) out[q] = wn;
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void kSelfIntersectionsCoop(MeshDev m, int nLogical, float beta,
+                                                                 uint8_t* __restrict__ out)
+{
+    __shared__ uint4 stkMem[kWaves][D];
+    const int lb = logicalBlock(nLogical);
+    if (lb >= nLogical) return;
+    const int k = lb * kBlock + threadIdx.x;
+    const bool valid = k < m.numFaces;
+    if (!waveBallot(valid)) return;
+    const Leaf L = loadLeaf(m.tri, valid ? k : 0);
+    const float third = 1 / 3.0f;
+    const float qx = third * ((L.a[0] + L.b[0]) + L.c[0]);
+    const float qy = third * ((L.a[1] + L.b[1]) + L.c[1]);
+    const float qz = third * ((L.a[2] + L.b[2]) + L.c[2]);
+    const float wn = coopWinding(m, qx, qy, qz, valid, beta, L.face, stkMem[threadIdx.x >> 6]);
+    if (valid) out[L.face] = (wn < 0 || wn > 1) ? 1 : 0;
+}
+
 __device__ __forceinline__ uint32_t spread10(uint32_t v)
 {
     v &= 0x3ffu;
@@ -447,6 +677,13 @@ int stackDepthFor(int treeDepth)
     return 0;
 }
 
+namespace {
+std::atomic<int> g_variant{ (int)Traversal::WaveCoop };
+}
+
+void setTraversal(Traversal t) { g_variant.store((int)t); }
+Traversal traversal() { return (Traversal)g_variant.load(); }
+
 hipError_t launchSdfGrid(const MeshDev& m, const int32_t dims[3], const Xf& xf, const DistOpts& o,
                          int32_t zBegin, int32_t zEnd, float* out, int stackD, hipStream_t s)
 {
@@ -454,10 +691,18 @@ hipError_t launchSdfGrid(const MeshDev& m, const int32_t dims[3], const Xf& xf, 
     const int64_t bricks = (int64_t)nbx * nby * nbz;
     if (bricks <= 0) return hipSuccess;
     const int nLogical = (int)((bricks + 3) / 4);
+    // the wave-cooperative distance returns the minimum over candidates; with minDistSq > 0 the
+    // reference stops at the first candidate <= minDistSq in its own visit order, so that case
+    // keeps the per-lane traversal
+    const bool coop = traversal() == Traversal::WaveCoop && !(o.minDistSq > 0);
     return dispatchDepth(stackD, [&](auto Dc) {
         constexpr int D = decltype(Dc)::value;
-        hipLaunchKernelGGL(kSdfGrid<D>, dim3(paddedGrid(nLogical)), dim3(kBlock), 0, s, m, dims[0],
-                           dims[1], zBegin, zEnd, nbx, nby, nbz, nLogical, xf, o, out);
+        if (coop)
+            hipLaunchKernelGGL(kSdfGridCoop<D>, dim3(paddedGrid(nLogical)), dim3(kBlock), 0, s, m, dims[0],
+                               dims[1], zBegin, zEnd, nbx, nby, nbz, nLogical, xf, o, out);
+        else
+            hipLaunchKernelGGL(kSdfGrid<D>, dim3(paddedGrid(nLogical)), dim3(kBlock), 0, s, m, dims[0],
+                               dims[1], zBegin, zEnd, nbx, nby, nbz, nLogical, xf, o, out);
         return hipGetLastError();
     });
 }
@@ -469,10 +714,15 @@ hipError_t launchFwnGrid(const MeshDev& m, const int32_t dims[3], const Xf& xf, 
     const int64_t bricks = (int64_t)nbx * nby * nbz;
     if (bricks <= 0) return hipSuccess;
     const int nLogical = (int)((bricks + 3) / 4);
+    const bool coop = traversal() == Traversal::WaveCoop;
     return dispatchDepth(stackD, [&](auto Dc) {
         constexpr int D = decltype(Dc)::value;
-        hipLaunchKernelGGL(kFwnGrid<D>, dim3(paddedGrid(nLogical)), dim3(kBlock), 0, s, m, dims[0],
-                           dims[1], zBegin, zEnd, nbx, nby, nbz, nLogical, xf, beta, out);
+        if (coop)
+            hipLaunchKernelGGL(kFwnGridCoop<D>, dim3(paddedGrid(nLogical)), dim3(kBlock), 0, s, m, dims[0],
+                               dims[1], zBegin, zEnd, nbx, nby, nbz, nLogical, xf, beta, out);
+        else
+            hipLaunchKernelGGL(kFwnGrid<D>, dim3(paddedGrid(nLogical)), dim3(kBlock), 0, s, m, dims[0],
+                               dims[1], zBegin, zEnd, nbx, nby, nbz, nLogical, xf, beta, out);
         return hipGetLastError();
     });
 }
@@ -482,10 +732,15 @@ hipError_t launchFwnPoints(const MeshDev& m, const float* pts, const uint32_t* p
 {
     if (n <= 0) return hipSuccess;
     const int nLogical = (int)((n + kBlock - 1) / kBlock);
+    const bool coop = traversal() == Traversal::WaveCoop;
     return dispatchDepth(stackD, [&](auto Dc) {
         constexpr int D = decltype(Dc)::value;
-        hipLaunchKernelGGL(kFwnPoints<D>, dim3(paddedGrid(nLogical)), dim3(kBlock), 0, s, m, pts,
-                           perm, n, nLogical, beta, skipFace, out);
+        if (coop)
+            hipLaunchKernelGGL(kFwnPointsCoop<D>, dim3(paddedGrid(nLogical)), dim3(kBlock), 0, s, m, pts,
+                               perm, n, nLogical, beta, skipFace, out);
+        else
+            hipLaunchKernelGGL(kFwnPoints<D>, dim3(paddedGrid(nLogical)), dim3(kBlock), 0, s, m, pts,
+                               perm, n, nLogical, beta, skipFace, out);
         return hipGetLastError();
     });
 }
@@ -494,10 +749,15 @@ hipError_t launchSelfIntersections(const MeshDev& m, float beta, uint8_t* out, i
 {
     if (m.numFaces <= 0) return hipSuccess;
     const int nLogical = (m.numFaces + kBlock - 1) / kBlock;
+    const bool coop = traversal() == Traversal::WaveCoop;
     return dispatchDepth(stackD, [&](auto Dc) {
         constexpr int D = decltype(Dc)::value;
-        hipLaunchKernelGGL(kSelfIntersections<D>, dim3(paddedGrid(nLogical)), dim3(kBlock), 0, s, m,
-                           nLogical, beta, out);
+        if (coop)
+            hipLaunchKernelGGL(kSelfIntersectionsCoop<D>, dim3(paddedGrid(nLogical)), dim3(kBlock), 0, s, m,
+                               nLogical, beta, out);
+        else
+            hipLaunchKernelGGL(kSelfIntersections<D>, dim3(paddedGrid(nLogical)), dim3(kBlock), 0, s, m,
+                               nLogical, beta, out);
         return hipGetLastError();
     });
 }

@@ -25,8 +25,11 @@ struct DistOpts { float minDistSq, maxDistSq; int32_t nullOutsideMinMax; float t
 struct ProjRec { int32_t face; float px, py, pz; int32_t e; float a, b; float distSq; };
 static_assert(sizeof(ProjRec) == 32, "MeshProjectionResult is 32 B");
 
-// kernel variant knobs chosen by the autotuner
-struct LaunchCfg { int blockSize = 256; int xcdRemap = 1; };
+// traversal form: PerLane = one independent depth-first walk per lane (the reference's form);
+// WaveCoop = one walk per wave over its 64 coherent queries (results identical, see kernels)
+enum class Traversal : int { PerLane = 0, WaveCoop = 1 };
+void setTraversal(Traversal t);
+Traversal traversal();
 
 // stack depth template actually used for a tree of the given depth (0 if unsupported)
 int stackDepthFor(int treeDepth);
