@@ -540,6 +540,137 @@ __device__ __forceinline__ float coopWinding(const MeshDev& m, float qx, float q
     return kInv4Pi * res;
 }
 
+// Closest point for each lane (findProjectionSubtree, source/MRMesh/MRMeshProject.cpp:13-103) on a
+// wave-shared walk.  The reference's answer is the leaf that minimises (distSq, k) where k is the
+// leaf's position in that lane's own depth-first order (nearer child first, ties to the right):
+// candidates are accepted only on a strict '<' and, while the best is still above the minimum,
+// no ancestor of the first minimal leaf can be pruned.  k is the lane's path of near(0)/far(1)
+// choices, MSB first (bit 31 - depth for a child one level below `depth`), so comparing keys
+// compares positions in the lane's order.  Pruning and acceptance both compare (distSq, key)
+// lexicographically, which makes the record bit-identical to the reference for any visit order.
+// Valid when loDistLimitSq <= 0 (the reference's early exit then only fires at distSq == 0,
+// which is the minimum) and for trees of depth <= 32.
+template <int D>
+__device__ __forceinline__ void coopProject(const MeshDev& m, float px, float py, float pz, float up, bool valid,
+                                            float4* stk, uint32_t* keys, BestHit& res)
+{
+    res.distSq = valid ? up : -1.f;
+    res.face = -1;
+    res.px = res.py = res.pz = 0;
+    res.a = res.b = 0;
+    uint32_t bestKey = 0;
+    if (m.numFaces <= 0) return;
+    const int lane = threadIdx.x & 63;
+    const double pd[3] = { px, py, pz };
+    int sp = 0;
+    int cur = m.rootRef, curDepth = 0;
+    float curd = boxDistSq(m.rootMin[0], m.rootMin[1], m.rootMin[2], m.rootMax[0], m.rootMax[1],
+                           m.rootMax[2], px, py, pz);
+    uint32_t curKey = 0;
+    bool have = true;
+    for (;;) {
+        if (!have) {
+            if (sp == 0) break;
+            --sp;
+            const float4 e0 = stk[2 * sp], e1 = stk[2 * sp + 1];
+            cur = __float_as_int(e0.x);
+            curDepth = __float_as_int(e1.w);
+            curd = boxDistSq(e0.y, e0.z, e0.w, e1.x, e1.y, e1.z, px, py, pz);
+            curKey = keys[sp * 64 + lane];
+        }
+        have = false;
+        const bool act = curd < res.distSq || (curd == res.distSq && curKey < bestKey);
+        if (!waveBallot(act)) continue;
+        cur = uniform(cur);
+        curDepth = uniform(curDepth);
+        if (cur < 0) {
+            const Leaf L = loadLeaf(m.tri, ~cur);
+            if (act) {
+                const double ad[3] = { L.a[0], L.a[1], L.a[2] };
+                const double bd[3] = { L.b[0], L.b[1], L.b[2] };
+                const double cd[3] = { L.c[0], L.c[1], L.c[2] };
+                double pr[3], ba, bb;
+                closestPointD(pd, ad, bd, cd, pr, ba, bb);
+                const float qx = (float)pr[0], qy = (float)pr[1], qz = (float)pr[2];
+                const float dx = qx - px, dy = qy - py, dz = qz - pz;
+                const float d = dx * dx + dy * dy + dz * dz;
+                if (d < res.distSq || (d == res.distSq && curKey < bestKey)) {
+                    res.distSq = d;
+                    bestKey = curKey;
+                    res.face = L.face;
+                    res.px = qx; res.py = qy; res.pz = qz;
+                    res.a = (float)ba; res.b = (float)bb;
+                }
+            }
+            continue;
+        }
+        const float4 n0 = m.dnode[4 * cur], n1 = m.dnode[4 * cur + 1];
+        const float4 n2 = m.dnode[4 * cur + 2], n3 = m.dnode[4 * cur + 3];
+        const float d1 = boxDistSq(n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, px, py, pz);
+        const float d2 = boxDistSq(n1.z, n1.w, n2.x, n2.y, n2.z, n2.w, px, py, pz);
+        const uint32_t bit = 1u << (31 - curDepth);
+        const bool leftNear = d1 < d2;                       // ties: right child first
+        const uint32_t kL = curKey | (leftNear ? 0u : bit), kR = curKey | (leftNear ? bit : 0u);
+        const bool aL = act && (d1 < res.distSq || (d1 == res.distSq && kL < bestKey));
+        const bool aR = act && (d2 < res.distSq || (d2 == res.distSq && kR < bestKey));
+        const uint64_t b1 = waveBallot(aL), b2 = waveBallot(aR);
+        if (!(b1 | b2)) continue;
+        bool leftFirst = b1 != 0;
+        if (b1 && b2) {
+            const int lead = __builtin_ctzll(b1 | b2);
+            leftFirst = laneValue(d1, lead) < laneValue(d2, lead);
+        }
+        const int childDepth = curDepth + 1;
+        if (leftFirst) {
+            if (b2) {
+                stk[2 * sp] = make_float4(n3.y, n1.z, n1.w, n2.x);
+                stk[2 * sp + 1] = make_float4(n2.y, n2.z, n2.w, __int_as_float(childDepth));
+                keys[sp * 64 + lane] = kR;
+                ++sp;
+            }
+            cur = __float_as_int(n3.x); curd = d1; curKey = kL;
+        } else {
+            if (b1) {
+                stk[2 * sp] = make_float4(n3.x, n0.x, n0.y, n0.z);
+                stk[2 * sp + 1] = make_float4(n0.w, n1.x, n1.y, __int_as_float(childDepth));
+                keys[sp * 64 + lane] = kL;
+                ++sp;
+            }
+            cur = __float_as_int(n3.y); curd = d2; curKey = kR;
+        }
+        curDepth = childDepth;
+        have = true;
+    }
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void kProjectionsCoop(MeshDev m, const float* __restrict__ pts,
+                                                           const uint32_t* __restrict__ perm, int64_t n,
+                                                           int nLogical, Xf ptXf, int hasPtXf, float up,
+                                                           ProjRec* __restrict__ out)
+{
+    __shared__ float4 stkMem[kWaves][2 * D];
+    __shared__ uint32_t keyMem[kWaves][D * 64];
+    const int lb = logicalBlock(nLogical);
+    if (lb >= nLogical) return;
+    const int64_t i = (int64_t)lb * kBlock + threadIdx.x;
+    const bool valid = i < n;
+    if (!waveBallot(valid)) return;
+    const int64_t q = valid ? (perm ? (int64_t)perm[i] : i) : 0;
+    float px = 0, py = 0, pz = 0;
+    if (valid) {
+        px = pts[3 * q]; py = pts[3 * q + 1]; pz = pts[3 * q + 2];
+        if (hasPtXf) xfApply(ptXf, px, py, pz);
+    }
+    const int w = threadIdx.x >> 6;
+    BestHit h;
+    coopProject<D>(m, px, py, pz, up, valid, stkMem[w], keyMem[w], h);
+    if (!valid) return;
+    float4* o = reinterpret_cast<float4*>(out + q);
+    o[0] = make_float4(__int_as_float(h.face), h.px, h.py, h.pz);
+    o[1] = make_float4(__int_as_float(-1), h.a, h.b, h.distSq);
+}
+
 template <int D>
 __global__ __launch_bounds__(kBlock) void kSdfGridCoop(MeshDev m, int dx, int dy, int zBegin, int zEnd,
                                                        int nbx, int nby, int nbz, int nLogical, Xf xf,
@@ -771,9 +902,15 @@ hipError_t launchProjections(const MeshDev& m, const float* pts, const uint32_t*
     Xf px{}, tx{};
     if (ptXf) px = *ptXf;
     if (treeXf) tx = *treeXf;
+    // the shared walk reproduces the reference's record exactly when its early exit cannot pick a
+    // non-minimal leaf (loDistLimitSq <= 0) and the tree is not transformed per box
+    const bool coop = traversal() == Traversal::WaveCoop && !(lo > 0) && !treeXf;
     return dispatchDepth(stackD, [&](auto Dc) {
         constexpr int D = decltype(Dc)::value;
-        if (treeXf)
+        if (coop)
+            hipLaunchKernelGGL(kProjectionsCoop<D>, dim3(paddedGrid(nLogical)), dim3(kBlock), 0, s, m, pts,
+                               perm, n, nLogical, px, ptXf ? 1 : 0, up, out);
+        else if (treeXf)
             hipLaunchKernelGGL((kProjections<true, D>), dim3(paddedGrid(nLogical)), dim3(kBlock), 0, s,
                                m, pts, perm, n, nLogical, px, ptXf ? 1 : 0, tx, up, lo, out);
         else

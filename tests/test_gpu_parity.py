@@ -154,6 +154,37 @@ def test_traversal_forms_identical(k5, per_lane_then_restore):
         )
     for a, b in zip(outs["per_lane"], outs["wave_coop"]):
         assert np.array_equal(_bits(a), _bits(b))
+    pr = M.PointsToMeshProjector()
+    pr.updateMeshData(gm)
+    refXf = np.array([1.3, 0, 0.1, 0, 0.8, 0, 0.05, 0, 1.1, 0.1, 0.0, -0.1], np.float32)
+    proj = {}
+    for mode in ("per_lane", "wave_coop"):
+        M.set_traversal(mode)
+        proj[mode] = (pr.findProjections(pts), pr.findProjections(pts, upDistLimitSq=0.02),
+                      pr.findProjections(pts, loDistLimitSq=0.01), pr.findProjections(pts, refObjXf=refXf))
+    for a, b in zip(proj["per_lane"], proj["wave_coop"]):
+        assert np.array_equal(_bits(a), _bits(b))
+
+
+def test_projection_ties(k5):
+    # points just outside the vertices (along the radial direction): the vertex is the closest
+    # feature of every incident face, so several faces tie at exactly the same distSq and the
+    # reference's depth-first order picks the winner
+    v, f, gm, om = k5
+    vv = v.astype(np.float64)
+    n = vv / np.linalg.norm(vv, axis=1, keepdims=True)
+    pts = np.concatenate([(vv + 0.003 * n), vv, 0.5 * (vv[f[:, 0]] + vv[f[:, 1]])]).astype(np.float32)
+    pr = M.PointsToMeshProjector()
+    pr.updateMeshData(gm)
+    o = om.find_projections(pts)
+    for mode in ("wave_coop", "per_lane"):
+        M.set_traversal(mode)
+        g = pr.findProjections(pts)
+        assert np.array_equal(_bits(g), _bits(o)), mode
+    M.set_traversal("wave_coop")
+    # the ties are real: many queries have another face at exactly the winning distance
+    d64 = O.bf_nearest_f64(v, f, pts[:2000])
+    assert np.mean(o["distSq"][:2000] > 0) > 0.9 and d64.shape[0] == 2000
 
 
 def test_chunked_and_by_parts(k5):
