@@ -31,6 +31,77 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+TRAIN_N, TRAIN_BATCH, TRAIN_LR, TRAIN_SEED = 1 << 20, 8192, 1e-3, 11
+MARGINS = (0.05, 0.1, 0.25)
+# FLOP per sample: forward 2*(99*128 + 3*128*128 + 128); backward 2 GEMMs per layer (dW, dH)
+MLP_FWD_FLOP = 2 * (99 * 128 + 3 * 128 * 128 + 128)
+MLP_BWD_FLOP = 2 * (99 * 128 + 3 * 128 * 128 + 128) + 2 * (3 * 128 * 128 + 128)
+
+
+def learned_leg(M, torch, np, synth, v, f, mesh, fwn, proj, dims, xf, opts, d_pts, d_sdf, d_proj,
+                res_sdf, res_proj, sdf_avg, proj_avg, plain, ev, stream, sh):
+    """Learned bound: train the MLP on labels from the GPU projector, then run projections and the
+    signed-distance grid with the capped search at each margin; outputs must be identical."""
+    NQ = d_pts.shape[0]
+    nvox = d_sdf.shape[0]
+    lo, hi = v.min(0).astype(np.float64), v.max(0).astype(np.float64)
+    net = M.DistanceMLP(0.5 * (lo + hi), float(0.5 * np.max(hi - lo) * 1.2), seed=TRAIN_SEED)
+    tp = torch.from_numpy(synth.projection_queries(v, f, TRAIN_N, TRAIN_SEED)).cuda()
+    trec = torch.empty(TRAIN_N * 8, dtype=torch.int32, device="cuda")
+    proj.find_projections_device(tp.data_ptr(), TRAIN_N, trec.data_ptr(), stream=sh)
+    tlab = trec.view(torch.float32).view(-1, 8)[:, 7].sqrt().contiguous()
+    torch.cuda.synchronize()
+    loss0 = net.loss(tp[:65536], tlab[:65536])
+    ev[0].record(stream)
+    net.train(tp, tlab, steps=STEPS_TRAIN[0], batch=TRAIN_BATCH, lr=TRAIN_LR, stream=sh)
+    ev[1].record(stream)
+    torch.cuda.synchronize()
+    train_ms = ev[0].elapsed_time(ev[1])
+    loss1 = net.loss(tp[:65536], tlab[:65536])
+    # held-out error of the prediction on the measured query set
+    ref_d = res_proj.view(torch.float32).view(-1, 8)[:, 7].double().sqrt().numpy()
+    ev[0].record(stream)
+    pred = net.predict(d_pts)
+    ev[1].record(stream)
+    torch.cuda.synchronize()
+    infer_ms = ev[0].elapsed_time(ev[1])
+    rel = np.abs(pred - ref_d) / np.maximum(ref_d, 1e-3)
+    out = {"train": {"samples": TRAIN_N, "steps": STEPS_TRAIN[0], "batch": TRAIN_BATCH, "lr": TRAIN_LR,
+                     "ms": train_ms, "loss_before": loss0, "loss_after": loss1,
+                     "tflops": STEPS_TRAIN[0] * TRAIN_BATCH * (MLP_FWD_FLOP + MLP_BWD_FLOP) / (train_ms / 1e3) / 1e12},
+           "inference": {"points": NQ, "ms": infer_ms,
+                         "tflops": NQ * MLP_FWD_FLOP / (infer_ms / 1e3) / 1e12,
+                         "median_rel_err": float(np.median(rel)), "p90_rel_err": float(np.quantile(rel, 0.9)),
+                         "under_prediction_rate": float(np.mean(pred < ref_d))},
+           "margins": {}}
+    base_step = sdf_avg + proj_avg
+    for m in MARGINS:
+        ev[0].record(stream)
+        rp = proj.find_projections_learned_device(net, m, d_pts.data_ptr(), NQ, d_proj.data_ptr(), stream=sh)
+        ev[1].record(stream)
+        torch.cuda.synchronize()
+        pm = ev[0].elapsed_time(ev[1])
+        same_p = bool(torch.equal(d_proj.cpu(), res_proj))
+        ev[0].record(stream)
+        rs = fwn.sdf_grid_learned_device(net, m, dims, xf, opts, d_sdf.data_ptr(), stream=sh)
+        ev[1].record(stream)
+        torch.cuda.synchronize()
+        sm = ev[0].elapsed_time(ev[1])
+        same_s = bool(torch.equal(d_sdf.cpu().view(torch.int32), res_sdf.view(torch.int32)))
+        e = {"proj_ms": pm, "sdf_ms": sm, "proj_fallback_rate": rp / NQ, "sdf_fallback_rate": rs / nvox,
+             "identical_projections": same_p, "identical_sdf": same_s,
+             "speedup_vs_plain_coop": base_step / (pm + sm)}
+        if plain:
+            e["speedup_vs_per_lane"] = (plain["sdf_ms"] + plain["proj_ms"]) / (pm + sm)
+        out["margins"][str(m)] = e
+        log(f"learned m={m}: proj {pm:.2f} ms (fallback {rp / NQ:.4f}), sdf {sm:.2f} ms "
+            f"(fallback {rs / nvox:.4f}), identical {same_p} {same_s}")
+    return out
+
+
+STEPS_TRAIN = [2000]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -42,7 +113,10 @@ def main():
                     help="kernel traversal form of the measured (value) run")
     ap.add_argument("--no-plain", action="store_true",
                     help="skip the one-step comparison run with the per-lane (reference-form) traversal")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-bound leg")
+    ap.add_argument("--train-steps", type=int, default=2000)
     args = ap.parse_args()
+    STEPS_TRAIN[0] = args.train_steps
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: this build has no multi-GPU path (asked for --gpus "
               f"{args.gpus}); the path shards by z-slabs / query ranges with no exchange, see DESIGN.md",
@@ -128,6 +202,11 @@ def main():
     pts_rate = NQ / (proj_avg / 1000.0)
     value = (nvox + NQ) / (ms_per_step / 1000.0)
 
+    learned = None
+    if not args.no_learned:
+        learned = learned_leg(M, torch, np, synth, v, f, mesh, fwn, proj, dims, xf, opts, d_pts, d_sdf, d_proj,
+                              res_sdf, res_proj, sdf_avg, proj_avg, plain, ev, stream, sh)
+
     # ---- CPU baseline (the oracle: C restatement of the reference CPU path) + algorithmic bytes
     cpu = None
     b_vox = b_pt = None
@@ -186,7 +265,7 @@ def main():
                    "points": NQ, "beta": 2.0},
         "voxels_per_s": vox_rate, "pts_per_s": pts_rate, "sdf_ms": sdf_avg, "proj_ms": proj_avg,
         "traversal": args.traversal, "vs_per_lane": plain,
-        "learned_vs_plain": None,
+        "learned_vs_plain": learned,
         "roofline": roof, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -167,6 +167,56 @@ int mi_proj_find_projections_device(mi_mesh* m, const float* d_pts, int64_t n, c
                                     const float* treeXf, float upDistLimitSq, float loDistLimitSq,
                                     mi_projection* d_res, void* stream);
 
+/* findProjection with a per-point upDistLimitSq: point i searches below min(d_upSq[i], upDistLimitSq),
+ * the pattern of ICP's limitedProjector (source/MRMesh/MRMeshOrPoints.cpp:121-143 calling
+ * findProjection, MRMeshProject.cpp:123-127 with upDistLimitSq = the previous distance). */
+int mi_proj_find_projections_bounded_device(mi_mesh* m, const float* d_pts, int64_t n, const float* d_upSq,
+                                            const float* ptXf, const float* treeXf, float upDistLimitSq,
+                                            float loDistLimitSq, mi_projection* d_res, void* stream);
+
+/* Exact projections accelerated by per-point caps (no counterpart in the reference; the learned
+ * bound of this project feeds it): a first pass searches below min(d_capSq[i], upDistLimitSq), the
+ * points that found nothing there are searched again without the cap.  The records equal
+ * findProjections(upDistLimitSq, loDistLimitSq = 0) whatever the caps; *reruns returns how many
+ * points needed the second pass.  Synchronises the stream once to size the second pass. */
+int mi_proj_find_projections_capped_device(mi_mesh* m, const float* d_pts, int64_t n, const float* d_capSq,
+                                           float upDistLimitSq, mi_projection* d_res, int64_t* reruns,
+                                           void* stream);
+/* The same for the signed-distance grid slab (calcFromGridWithDistances with minDistSq <= 0):
+ * d_capSq holds one cap per output voxel of the slab. */
+int mi_sdf_grid_capped_device(mi_mesh* m, const int32_t dims[3], const float xf[12], const mi_dist_options* opts,
+                              const float* d_capSq, int32_t zBegin, int32_t zEnd, float* d_res,
+                              int64_t* reruns, void* stream);
+
+/* ---------------------------------------------------------------- learned distance bound
+ * No counterpart in the reference: a Fourier-feature MLP (99 features -> 4 x 128 SiLU -> softplus)
+ * predicts each query's unsigned distance; the prediction caps the search radius at
+ * (pred * (1 + margin))^2 and the capped forms above keep the outputs exact.  Trained and evaluated
+ * on the GPU with BF16 MFMA kernels accumulating in FP32. */
+typedef struct mi_mlp mi_mlp;
+/* network for points around `center`, coordinates normalised by `scale` (> 0); seeded Xavier init */
+int  mi_mlp_create(const float center[3], float scale, uint64_t seed, mi_mlp** out);
+void mi_mlp_destroy(mi_mlp* m);
+int  mi_mlp_num_params(void);
+/* FP32 parameter vector: W_l [4][128 out][128 in], b_l [4][128], w5 [128], b5 */
+int  mi_mlp_get_params(mi_mlp* m, float* host);
+int  mi_mlp_set_params(mi_mlp* m, const float* host);
+/* gradient of 1/(2n) sum_i (pred_i - label_i)^2 / scale^2 over the n points, into d_grad[params] */
+int  mi_mlp_gradient_device(mi_mlp* m, const float* d_pts, const float* d_labels, int64_t n, float* d_grad,
+                            void* stream);
+/* `steps` Adam steps (lr, beta 0.9 / 0.999, eps 1e-8) over consecutive batches of d_pts/d_labels */
+int  mi_mlp_train_device(mi_mlp* m, const float* d_pts, const float* d_labels, int64_t n, int32_t steps,
+                         int32_t batch, float lr, void* stream);
+/* predicted unsigned distance per point */
+int  mi_mlp_predict_device(mi_mlp* m, const float* d_pts, int64_t n, float* d_dist, void* stream);
+/* findProjections / calcFromGridWithDistances with the learned cap; exact (see the capped forms) */
+int  mi_proj_find_projections_learned_device(mi_mesh* mesh, mi_mlp* m, float margin, const float* d_pts,
+                                             int64_t n, float upDistLimitSq, mi_projection* d_res,
+                                             int64_t* reruns, void* stream);
+int  mi_sdf_grid_learned_device(mi_mesh* mesh, mi_mlp* m, float margin, const int32_t dims[3], const float xf[12],
+                                const mi_dist_options* opts, int32_t zBegin, int32_t zEnd, float* d_res,
+                                int64_t* reruns, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -24,7 +24,7 @@ from typing import Callable, Optional
 
 import numpy as np
 
-__all__ = ["Mesh", "FastWindingNumber", "PointsToMeshProjector", "DistanceToMeshOptions",
+__all__ = ["Mesh", "FastWindingNumber", "PointsToMeshProjector", "DistanceToMeshOptions", "DistanceMLP",
            "MeshLibAmdError", "PROJ_DTYPE", "NODE_DTYPE", "DIPOLE_DTYPE", "lib", "lib_path",
            "set_memory_limit", "FLT_MAX"]
 
@@ -112,6 +112,19 @@ def lib():
         "mi_proj_find_projections": (C.c_int, [P, P, I64, P, P, F, F, P]),
         "mi_proj_heap_bytes": (SZ, [SZ]),
         "mi_proj_find_projections_device": (C.c_int, [P, P, I64, P, P, F, F, P, P]),
+        "mi_proj_find_projections_bounded_device": (C.c_int, [P, P, I64, P, P, P, F, F, P, P]),
+        "mi_proj_find_projections_capped_device": (C.c_int, [P, P, I64, P, F, P, C.POINTER(I64), P]),
+        "mi_sdf_grid_capped_device": (C.c_int, [P, P, P, P, P, I32, I32, P, C.POINTER(I64), P]),
+        "mi_mlp_create": (C.c_int, [P, F, C.c_uint64, C.POINTER(P)]),
+        "mi_mlp_destroy": (None, [P]),
+        "mi_mlp_num_params": (C.c_int, []),
+        "mi_mlp_get_params": (C.c_int, [P, P]),
+        "mi_mlp_set_params": (C.c_int, [P, P]),
+        "mi_mlp_gradient_device": (C.c_int, [P, P, P, I64, P, P]),
+        "mi_mlp_train_device": (C.c_int, [P, P, P, I64, I32, I32, F, P]),
+        "mi_mlp_predict_device": (C.c_int, [P, P, I64, P, P]),
+        "mi_proj_find_projections_learned_device": (C.c_int, [P, P, F, P, I64, F, P, C.POINTER(I64), P]),
+        "mi_sdf_grid_learned_device": (C.c_int, [P, P, F, P, P, P, I32, I32, P, C.POINTER(I64), P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -249,6 +262,69 @@ class Mesh:
         return int(lib().mi_mesh_depth(self._h))
 
 
+class DistanceMLP:
+    """The learned distance bound: Fourier-feature MLP (99 -> 4 x 128 SiLU -> softplus) trained and
+    evaluated on the GPU (BF16 MFMA kernels, FP32 accumulation).  Device arrays are torch tensors."""
+
+    def __init__(self, center, scale: float, seed: int = 0):
+        c = np.ascontiguousarray(center, dtype=np.float32).reshape(3)
+        h = C.c_void_p()
+        _check(lib().mi_mlp_create(_p(c), float(scale), int(seed), C.byref(h)))
+        self._h = h
+        self.center, self.scale = c, float(scale)
+        self.num_params = int(lib().mi_mlp_num_params())
+
+    @property
+    def handle(self):
+        return self._h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                lib().mi_mlp_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    def get_params(self) -> np.ndarray:
+        out = np.zeros(self.num_params, np.float32)
+        _check(lib().mi_mlp_get_params(self._h, _p(out)))
+        return out
+
+    def set_params(self, theta):
+        th = np.ascontiguousarray(theta, dtype=np.float32).reshape(-1)
+        assert th.size == self.num_params
+        _check(lib().mi_mlp_set_params(self._h, _p(th)))
+
+    def predict(self, d_pts, stream: int = 0):
+        import torch
+        n = d_pts.shape[0]
+        out = torch.empty(n, dtype=torch.float32, device=d_pts.device)
+        _check(lib().mi_mlp_predict_device(self._h, C.c_void_p(d_pts.data_ptr()), n, C.c_void_p(out.data_ptr()),
+                                           C.c_void_p(stream or None)))
+        torch.cuda.synchronize()
+        return out.cpu().numpy()
+
+    def gradient(self, d_pts, d_labels, stream: int = 0):
+        import torch
+        g = torch.zeros(self.num_params + 256, dtype=torch.float32, device=d_pts.device)
+        _check(lib().mi_mlp_gradient_device(self._h, C.c_void_p(d_pts.data_ptr()), C.c_void_p(d_labels.data_ptr()),
+                                            d_pts.shape[0], C.c_void_p(g.data_ptr()), C.c_void_p(stream or None)))
+        torch.cuda.synchronize()
+        return g.cpu().numpy()[:self.num_params]
+
+    def loss(self, d_pts, d_labels) -> float:
+        pred = self.predict(d_pts)
+        lab = d_labels.cpu().numpy()
+        d = (pred - lab) / self.scale
+        return float(0.5 * np.mean(d.astype(np.float64) ** 2))
+
+    def train(self, d_pts, d_labels, steps: int, batch: int, lr: float, stream: int = 0):
+        _check(lib().mi_mlp_train_device(self._h, C.c_void_p(d_pts.data_ptr()), C.c_void_p(d_labels.data_ptr()),
+                                         d_pts.shape[0], steps, batch, lr, C.c_void_p(stream or None)))
+
+
 class FastWindingNumber:
     """``MR::IFastWindingNumber`` / ``IFastWindingNumberByParts`` on the GPU."""
 
@@ -330,6 +406,31 @@ class FastWindingNumber:
         _check(lib().mi_fwn_grid_device(self.mesh.handle, _p(d), _p(_xf(xf)), beta, zBegin, zEnd,
                                         C.c_void_p(d_out), C.c_void_p(stream or None)))
 
+    def sdf_grid_capped_device(self, dims, xf, options, d_capSq: int, d_out: int, zBegin=0, zEnd=None,
+                               stream: int = 0) -> int:
+        """Exact signed-distance slab with per-voxel caps on the search radius (squared); returns how
+        many voxels needed the uncapped second pass."""
+        d = _dims(dims)
+        o = (options or DistanceToMeshOptions())._c()
+        zEnd = int(d[2]) if zEnd is None else zEnd
+        r = C.c_int64(0)
+        _check(lib().mi_sdf_grid_capped_device(self.mesh.handle, _p(d), _p(_xf(xf)), C.byref(o),
+                                               C.c_void_p(d_capSq), zBegin, zEnd, C.c_void_p(d_out),
+                                               C.byref(r), C.c_void_p(stream or None)))
+        return int(r.value)
+
+    def sdf_grid_learned_device(self, net: "DistanceMLP", margin: float, dims, xf, options, d_out: int,
+                                zBegin=0, zEnd=None, stream: int = 0) -> int:
+        """Exact signed-distance slab with the learned cap (pred * (1 + margin))^2; returns reruns."""
+        d = _dims(dims)
+        o = (options or DistanceToMeshOptions())._c()
+        zEnd = int(d[2]) if zEnd is None else zEnd
+        r = C.c_int64(0)
+        _check(lib().mi_sdf_grid_learned_device(self.mesh.handle, net.handle, margin, _p(d), _p(_xf(xf)),
+                                                C.byref(o), zBegin, zEnd, C.c_void_p(d_out), C.byref(r),
+                                                C.c_void_p(stream or None)))
+        return int(r.value)
+
     def fwn_vector_device(self, d_pts: int, n: int, beta: float, d_out: int, skipFace: int = -1,
                           stream: int = 0):
         _check(lib().mi_fwn_vector_device(self.mesh.handle, C.c_void_p(d_pts), n, beta, skipFace,
@@ -354,6 +455,35 @@ class PointsToMeshProjector:
         _check(lib().mi_proj_find_projections(self.mesh.handle, _p(pts), pts.shape[0], _p(_xf(objXf)),
                                               _p(_xf(refObjXf)), upDistLimitSq, loDistLimitSq, _p(res)))
         return res
+
+    def find_projections_bounded_device(self, d_pts: int, n: int, d_upSq: int, d_out: int, ptXf=None,
+                                        treeXf=None, upDistLimitSq: float = FLT_MAX,
+                                        loDistLimitSq: float = 0.0, stream: int = 0):
+        """findProjection with a per-point upDistLimitSq (ICP's limitedProjector pattern)."""
+        _check(lib().mi_proj_find_projections_bounded_device(
+            self.mesh.handle, C.c_void_p(d_pts), n, C.c_void_p(d_upSq), _p(_xf(ptXf)), _p(_xf(treeXf)),
+            upDistLimitSq, loDistLimitSq, C.c_void_p(d_out), C.c_void_p(stream or None)))
+
+    def find_projections_capped_device(self, d_pts: int, n: int, d_capSq: int, d_out: int,
+                                       upDistLimitSq: float = FLT_MAX, stream: int = 0) -> int:
+        """Exact projections with per-point caps on the search radius (squared); returns how many
+        points needed the uncapped second pass."""
+        r = C.c_int64(0)
+        _check(lib().mi_proj_find_projections_capped_device(
+            self.mesh.handle, C.c_void_p(d_pts), n, C.c_void_p(d_capSq), upDistLimitSq, C.c_void_p(d_out),
+            C.byref(r), C.c_void_p(stream or None)))
+        return int(r.value)
+
+    def find_projections_learned_device(self, net: "DistanceMLP", margin: float, d_pts: int, n: int,
+                                        d_out: int, upDistLimitSq: float = FLT_MAX, stream: int = 0) -> int:
+        """Exact projections with the learned cap (pred * (1 + margin))^2; returns how many points
+        were searched again without the cap."""
+        r = C.c_int64(0)
+        _check(lib().mi_proj_find_projections_learned_device(self.mesh.handle, net.handle, margin,
+                                                             C.c_void_p(d_pts), n, upDistLimitSq,
+                                                             C.c_void_p(d_out), C.byref(r),
+                                                             C.c_void_p(stream or None)))
+        return int(r.value)
 
     def projectionsHeapBytes(self, numProjections: int) -> int:
         return int(lib().mi_proj_heap_bytes(numProjections))

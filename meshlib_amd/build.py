@@ -21,7 +21,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmeshlib_amd.so")
 ARCH = os.environ.get("MESHLIB_AMD_ARCH", "gfx950")
 
-SOURCES = ["mi_tree.cpp", "mi_kernels.hip", "mi_capi.cpp"]
+SOURCES = ["mi_tree.cpp", "mi_kernels.hip", "mi_capi.cpp", "mi_mlp.hip", "mi_mlp_capi.cpp"]
 
 
 def hipcc() -> str:
@@ -49,5 +49,23 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+ADAPTER_SRC = os.path.join(ROOT, "tests", "cpp", "adapter_main.cpp")
+ADAPTER_BIN = os.path.join(ROOT, "tests", "cpp", "adapter_main")
+
+
+def build_adapter_driver(verbose: bool = False) -> str:
+    """Host-only C++ program driving the drop-in classes of MRAmdDistanceField.h (linked against
+    the library built above; runs on the GPU box from the GPU tests)."""
+    cxx = shutil.which("g++") or shutil.which("c++")
+    cmd = [cxx, "-std=c++17", "-O2", "-Wall", "-DMESHLIB_AMD_USE_MIRROR", "-I", os.path.join(ROOT, "include"),
+           ADAPTER_SRC, "-o", ADAPTER_BIN, "-L", HERE, "-lmeshlib_amd",
+           "-Wl,-rpath,$ORIGIN/../../meshlib_amd"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    return ADAPTER_BIN
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_adapter_driver(verbose=True))

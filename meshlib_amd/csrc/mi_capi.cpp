@@ -148,6 +148,9 @@ const float kIdentityXf[12] = { 1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0 };
 
 } // namespace
 
+// error reporting shared with the other translation units of the library
+int mi_internal_fail(int code, const char* msg) { return fail(code, msg); }
+
 struct mi_mesh {
     int device = 0;
     int32_t nv = 0, nf = 0;
@@ -375,6 +378,88 @@ int mi_proj_find_projections_device(mi_mesh* m, const float* d_pts, int64_t n, c
     if (treeXf) tx = toXf(treeXf);
     MI_HIP(mi::launchProjections(m->dev, d_pts, perm, n, ptXf ? &px : nullptr, treeXf ? &tx : nullptr, up, lo,
                                  reinterpret_cast<mi::ProjRec*>(d_res), m->stackD, s));
+    return MI_OK;
+}
+
+int mi_proj_find_projections_bounded_device(mi_mesh* m, const float* d_pts, int64_t n, const float* d_upSq,
+                                            const float* ptXf, const float* treeXf, float up, float lo,
+                                            mi_projection* d_res, void* stream)
+{
+    if (!m || n < 0 || (n > 0 && (!d_pts || !d_res || !d_upSq))) return fail(MI_E_INVALID, "invalid arguments");
+    if (n > INT32_MAX) return fail(MI_E_LIMIT, "more than 2^31-1 points in one call");
+    if (n == 0) return MI_OK;
+    std::lock_guard<std::mutex> lk(m->mu);
+    MI_HIP(hipSetDevice(m->device));
+    hipStream_t s = pick(m, stream);
+    if (int rc = ensureWorkspace(m, mi::sortWorkspaceBytes(n))) return rc;
+    const uint32_t* perm = nullptr;
+    MI_HIP(mi::sortPointsMorton(d_pts, n, m->sortLo, m->sortHi, m->ws.p, m->ws.bytes, &perm, s));
+    mi::Xf px, tx;
+    if (ptXf) px = toXf(ptXf);
+    if (treeXf) tx = toXf(treeXf);
+    MI_HIP(mi::launchProjections(m->dev, d_pts, perm, n, ptXf ? &px : nullptr, treeXf ? &tx : nullptr, up, lo,
+                                 reinterpret_cast<mi::ProjRec*>(d_res), m->stackD, s, d_upSq));
+    return MI_OK;
+}
+
+int mi_proj_find_projections_capped_device(mi_mesh* m, const float* d_pts, int64_t n, const float* d_capSq,
+                                           float up, mi_projection* d_res, int64_t* reruns, void* stream)
+{
+    if (!m || n < 0 || (n > 0 && (!d_pts || !d_res || !d_capSq))) return fail(MI_E_INVALID, "invalid arguments");
+    if (n > INT32_MAX) return fail(MI_E_LIMIT, "more than 2^31-1 points in one call");
+    if (reruns) *reruns = 0;
+    if (n == 0) return MI_OK;
+    std::lock_guard<std::mutex> lk(m->mu);
+    MI_HIP(hipSetDevice(m->device));
+    hipStream_t s = pick(m, stream);
+    const size_t sortBytes = align256(mi::sortWorkspaceBytes(n));
+    if (int rc = ensureWorkspace(m, sortBytes + align256((size_t)n * 4) + 256)) return rc;
+    char* base = m->ws.as<char>();
+    uint32_t* list = reinterpret_cast<uint32_t*>(base + sortBytes);
+    uint32_t* count = reinterpret_cast<uint32_t*>(base + sortBytes + align256((size_t)n * 4));
+    const uint32_t* perm = nullptr;
+    MI_HIP(mi::sortPointsMorton(d_pts, n, m->sortLo, m->sortHi, m->ws.p, sortBytes, &perm, s));
+    MI_HIP(mi::fillU32(count, 1, 0u, s));
+    MI_HIP(mi::launchProjections(m->dev, d_pts, perm, n, nullptr, nullptr, up, 0.f,
+                                 reinterpret_cast<mi::ProjRec*>(d_res), m->stackD, s, d_capSq, list, count));
+    uint32_t h = 0;
+    MI_HIP(hipMemcpyAsync(&h, count, 4, hipMemcpyDeviceToHost, s));
+    MI_HIP(hipStreamSynchronize(s));
+    if (h)
+        MI_HIP(mi::launchProjections(m->dev, d_pts, list, (int64_t)h, nullptr, nullptr, up, 0.f,
+                                     reinterpret_cast<mi::ProjRec*>(d_res), m->stackD, s));
+    if (reruns) *reruns = h;
+    return MI_OK;
+}
+
+int mi_sdf_grid_capped_device(mi_mesh* m, const int32_t dims[3], const float xf[12], const mi_dist_options* opts,
+                              const float* d_capSq, int32_t zBegin, int32_t zEnd, float* d_res, int64_t* reruns,
+                              void* stream)
+{
+    if (!m || !opts || !d_capSq) return fail(MI_E_INVALID, "mesh, options and caps are required");
+    if (int rc = checkGridArgs(dims, xf)) return rc;
+    if (zBegin < 0 || zEnd > dims[2] || zBegin > zEnd) return fail(MI_E_INVALID, "bad z-layer range");
+    if (opts->minDistSq > 0) return fail(MI_E_INVALID, "capped signed distance needs minDistSq <= 0");
+    if (reruns) *reruns = 0;
+    const size_t nvox = (size_t)dims[0] * dims[1] * (size_t)(zEnd - zBegin);
+    if (nvox == 0) return MI_OK;
+    if (!d_res) return fail(MI_E_INVALID, "output is null");
+    if (nvox > UINT32_MAX) return fail(MI_E_LIMIT, "slab larger than 2^32 voxels");
+    std::lock_guard<std::mutex> lk(m->mu);
+    MI_HIP(hipSetDevice(m->device));
+    hipStream_t s = pick(m, stream);
+    if (int rc = ensureWorkspace(m, align256(nvox * 4) + 256)) return rc;
+    uint32_t* list = m->ws.as<uint32_t>();
+    uint32_t* count = reinterpret_cast<uint32_t*>(m->ws.as<char>() + align256(nvox * 4));
+    mi::DistOpts o{ opts->minDistSq, opts->maxDistSq, opts->nullOutsideMinMax, opts->windingNumberThreshold,
+                    opts->windingNumberBeta };
+    MI_HIP(mi::fillU32(count, 1, 0u, s));
+    MI_HIP(mi::launchSdfGrid(m->dev, dims, toXf(xf), o, zBegin, zEnd, d_res, m->stackD, s, d_capSq, list, count));
+    uint32_t h = 0;
+    MI_HIP(hipMemcpyAsync(&h, count, 4, hipMemcpyDeviceToHost, s));
+    MI_HIP(hipStreamSynchronize(s));
+    if (h) MI_HIP(mi::launchSdfVoxels(m->dev, dims, toXf(xf), o, zBegin, list, (int64_t)h, d_res, m->stackD, s));
+    if (reruns) *reruns = h;
     return MI_OK;
 }
 

@@ -383,7 +383,8 @@ template <bool XF, int D>
 __global__ __launch_bounds__(kBlock) void kProjections(MeshDev m, const float* __restrict__ pts,
                                                        const uint32_t* __restrict__ perm, int64_t n,
                                                        int nLogical, Xf ptXf, int hasPtXf, Xf treeXf,
-                                                       float up, float lo, ProjRec* __restrict__ out)
+                                                       float up, float lo, const float* __restrict__ bound,
+                                                       ProjRec* __restrict__ out)
 {
     __shared__ int2 smem[D * kBlock];
     const int lb = logicalBlock(nLogical);
@@ -393,9 +394,11 @@ __global__ __launch_bounds__(kBlock) void kProjections(MeshDev m, const float* _
     const int64_t q = perm ? (int64_t)perm[i] : i;
     float px = pts[3 * q], py = pts[3 * q + 1], pz = pts[3 * q + 2];
     if (hasPtXf) xfApply(ptXf, px, py, pz);
+    float lim = up;
+    if (bound) { const float b = bound[q]; lim = b < up ? b : up; }
     BestHit h;
     h.px = h.py = h.pz = 0; h.a = h.b = 0;
-    distanceDfs<true, XF, D>(m, px, py, pz, up, lo, treeXf, smem + threadIdx.x, h);
+    distanceDfs<true, XF, D>(m, px, py, pz, lim, lo, treeXf, smem + threadIdx.x, h);
     float4* o = reinterpret_cast<float4*>(out + q);
     o[0] = make_float4(__int_as_float(h.face), h.px, h.py, h.pz);
     o[1] = make_float4(__int_as_float(-1), h.a, h.b, h.distSq);
@@ -643,10 +646,30 @@ __device__ __forceinline__ void coopProject(const MeshDev& m, float px, float py
     }
 }
 
+// appends `value` of every lane with p set to list[] (one atomic per wave)
+__device__ __forceinline__ void appendWave(bool p, uint32_t value, uint32_t* list, uint32_t* count)
+{
+    const uint64_t mask = waveBallot(p);
+    if (!mask) return;
+    const int lane = threadIdx.x & 63;
+    const int leader = __builtin_ctzll(mask);
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(mask));
+    base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
+    if (p) list[base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = value;
+}
+
+// bound (optional, indexed like the output): a per-query cap on upDistLimitSq.  With rerun lists
+// given, a query whose cap was below `up` and that found nothing below the cap is not written but
+// appended to the list, to be searched again without the cap (any cap above the true distance
+// gives the reference's record, so the combination is exact).
 template <int D>
 __global__ __launch_bounds__(kBlock) void kProjectionsCoop(MeshDev m, const float* __restrict__ pts,
                                                            const uint32_t* __restrict__ perm, int64_t n,
                                                            int nLogical, Xf ptXf, int hasPtXf, float up,
+                                                           const float* __restrict__ bound,
+                                                           uint32_t* __restrict__ rerun,
+                                                           uint32_t* __restrict__ rerunCount,
                                                            ProjRec* __restrict__ out)
 {
     __shared__ float4 stkMem[kWaves][2 * D];
@@ -658,23 +681,46 @@ __global__ __launch_bounds__(kBlock) void kProjectionsCoop(MeshDev m, const floa
     if (!waveBallot(valid)) return;
     const int64_t q = valid ? (perm ? (int64_t)perm[i] : i) : 0;
     float px = 0, py = 0, pz = 0;
+    float lim = up;
     if (valid) {
         px = pts[3 * q]; py = pts[3 * q + 1]; pz = pts[3 * q + 2];
         if (hasPtXf) xfApply(ptXf, px, py, pz);
+        if (bound) { const float b = bound[q]; lim = b < up ? b : up; }
     }
     const int w = threadIdx.x >> 6;
     BestHit h;
-    coopProject<D>(m, px, py, pz, up, valid, stkMem[w], keyMem[w], h);
-    if (!valid) return;
+    coopProject<D>(m, px, py, pz, lim, valid, stkMem[w], keyMem[w], h);
+    const bool again = rerun && valid && h.face < 0 && lim < up;
+    if (rerun) appendWave(again, (uint32_t)q, rerun, rerunCount);
+    if (!valid || again) return;
     float4* o = reinterpret_cast<float4*>(out + q);
     o[0] = make_float4(__int_as_float(h.face), h.px, h.py, h.pz);
     o[1] = make_float4(__int_as_float(-1), h.a, h.b, h.distSq);
 }
 
+// shared tail of the signed-distance kernels: NaN band, winding-number sign, store
+// (FastWindingNumber::calcWithDistances, source/MRMesh/MRFastWindingNumber.cpp:69-76)
+__device__ __forceinline__ void sdfStore(const MeshDev& m, float px, float py, float pz, bool valid, float resSq,
+                                         const DistOpts& o, uint4* stk, float* out, size_t idx)
+{
+    const bool nan = o.nullOutsideMinMax && (resSq < o.minDistSq || resSq >= o.maxDistSq);
+    const float wn = coopWinding(m, px, py, pz, valid && !nan, o.beta, -1, stk);
+    if (!valid) return;
+    if (nan) {
+        out[idx] = __int_as_float(0x7fc00000);
+        return;
+    }
+    const float sign = wn > o.threshold ? -1.f : +1.f;
+    out[idx] = sign * sqrtf(resSq);
+}
+
 template <int D>
 __global__ __launch_bounds__(kBlock) void kSdfGridCoop(MeshDev m, int dx, int dy, int zBegin, int zEnd,
                                                        int nbx, int nby, int nbz, int nLogical, Xf xf,
-                                                       DistOpts o, float* __restrict__ out)
+                                                       DistOpts o, const float* __restrict__ bound,
+                                                       uint32_t* __restrict__ rerun,
+                                                       uint32_t* __restrict__ rerunCount,
+                                                       float* __restrict__ out)
 {
     __shared__ float4 stkMem[kWaves][2 * D];
     const int lb = logicalBlock(nLogical);
@@ -687,19 +733,37 @@ __global__ __launch_bounds__(kBlock) void kSdfGridCoop(MeshDev m, int dx, int dy
     if (!waveBallot(valid)) return;
     float px = (float)x, py = (float)y, pz = (float)z;
     xfApply(xf, px, py, pz);
+    const size_t idx = valid ? ((size_t)(z - zBegin) * dy + y) * dx + x : 0;
+    float lim = o.maxDistSq;
+    if (bound && valid) { const float b = bound[idx]; lim = b < lim ? b : lim; }
+    const float resSq = coopDistance<D>(m, px, py, pz, valid ? lim : -1.f, stkMem[w]);
+    const bool again = rerun && valid && !(resSq < lim) && lim < o.maxDistSq;
+    if (rerun) appendWave(again, (uint32_t)idx, rerun, rerunCount);
+    sdfStore(m, px, py, pz, valid && !again, resSq, o, reinterpret_cast<uint4*>(stkMem[w]), out, idx);
+}
+
+// the listed voxels of a z-slab (ids relative to the slab), searched without any cap
+template <int D>
+__global__ __launch_bounds__(kBlock) void kSdfVoxelsCoop(MeshDev m, int dx, int dy, int zBegin,
+                                                         const uint32_t* __restrict__ list, int64_t count,
+                                                         int nLogical, Xf xf, DistOpts o,
+                                                         float* __restrict__ out)
+{
+    __shared__ float4 stkMem[kWaves][2 * D];
+    const int lb = logicalBlock(nLogical);
+    if (lb >= nLogical) return;
+    const int64_t i = (int64_t)lb * kBlock + threadIdx.x;
+    const bool valid = i < count;
+    if (!waveBallot(valid)) return;
+    const size_t idx = valid ? list[i] : 0;
+    const size_t layer = (size_t)dx * dy;
+    const int z = zBegin + (int)(idx / layer);
+    const int rem = (int)(idx % layer);
+    float px = (float)(rem % dx), py = (float)(rem / dx), pz = (float)z;
+    xfApply(xf, px, py, pz);
+    const int w = threadIdx.x >> 6;
     const float resSq = coopDistance<D>(m, px, py, pz, valid ? o.maxDistSq : -1.f, stkMem[w]);
-    // FastWindingNumber::calcWithDistances (source/MRMesh/MRFastWindingNumber.cpp:69-76)
-    const bool nan = o.nullOutsideMinMax && (resSq < o.minDistSq || resSq >= o.maxDistSq);
-    const float wn = coopWinding(m, px, py, pz, valid && !nan, o.beta, -1,
-                                 reinterpret_cast<uint4*>(stkMem[w]));
-    if (!valid) return;
-    const size_t idx = ((size_t)(z - zBegin) * dy + y) * dx + x;
-    if (nan) {
-        out[idx] = __int_as_float(0x7fc00000);
-        return;
-    }
-    const float sign = wn > o.threshold ? -1.f : +1.f;
-    out[idx] = sign * sqrtf(resSq);
+    sdfStore(m, px, py, pz, valid, resSq, o, reinterpret_cast<uint4*>(stkMem[w]), out, idx);
 }
 
 template <int D>
@@ -798,6 +862,21 @@ hipError_t dispatchDepth(int D, Fn&& fn)
 
 } // namespace
 
+namespace {
+__global__ void kFillU32(uint32_t* p, int n, uint32_t v)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+} // namespace
+
+hipError_t fillU32(uint32_t* p, int n, uint32_t v, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(kFillU32, dim3((n + 255) / 256), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
 int stackDepthFor(int treeDepth)
 {
     // pending far children of the ancestors of the current node: at most treeDepth entries
@@ -816,7 +895,8 @@ void setTraversal(Traversal t) { g_variant.store((int)t); }
 Traversal traversal() { return (Traversal)g_variant.load(); }
 
 hipError_t launchSdfGrid(const MeshDev& m, const int32_t dims[3], const Xf& xf, const DistOpts& o,
-                         int32_t zBegin, int32_t zEnd, float* out, int stackD, hipStream_t s)
+                         int32_t zBegin, int32_t zEnd, float* out, int stackD, hipStream_t s,
+                         const float* bound, uint32_t* rerun, uint32_t* rerunCount)
 {
     const int nbx = (dims[0] + 3) / 4, nby = (dims[1] + 3) / 4, nbz = (zEnd - zBegin + 3) / 4;
     const int64_t bricks = (int64_t)nbx * nby * nbz;
@@ -825,15 +905,32 @@ hipError_t launchSdfGrid(const MeshDev& m, const int32_t dims[3], const Xf& xf, 
     // the wave-cooperative distance returns the minimum over candidates; with minDistSq > 0 the
     // reference stops at the first candidate <= minDistSq in its own visit order, so that case
     // keeps the per-lane traversal
-    const bool coop = traversal() == Traversal::WaveCoop && !(o.minDistSq > 0);
+    const bool coop = (traversal() == Traversal::WaveCoop || bound) && !(o.minDistSq > 0);
+    if (bound && !coop) return hipErrorInvalidValue;
     return dispatchDepth(stackD, [&](auto Dc) {
         constexpr int D = decltype(Dc)::value;
         if (coop)
             hipLaunchKernelGGL(kSdfGridCoop<D>, dim3(paddedGrid(nLogical)), dim3(kBlock), 0, s, m, dims[0],
-                               dims[1], zBegin, zEnd, nbx, nby, nbz, nLogical, xf, o, out);
+                               dims[1], zBegin, zEnd, nbx, nby, nbz, nLogical, xf, o, bound, rerun, rerunCount,
+                               out);
         else
             hipLaunchKernelGGL(kSdfGrid<D>, dim3(paddedGrid(nLogical)), dim3(kBlock), 0, s, m, dims[0],
                                dims[1], zBegin, zEnd, nbx, nby, nbz, nLogical, xf, o, out);
+        return hipGetLastError();
+    });
+}
+
+hipError_t launchSdfVoxels(const MeshDev& m, const int32_t dims[3], const Xf& xf, const DistOpts& o,
+                           int32_t zBegin, const uint32_t* list, int64_t count, float* out, int stackD,
+                           hipStream_t s)
+{
+    if (count <= 0) return hipSuccess;
+    if (o.minDistSq > 0) return hipErrorInvalidValue;
+    const int nLogical = (int)((count + kBlock - 1) / kBlock);
+    return dispatchDepth(stackD, [&](auto Dc) {
+        constexpr int D = decltype(Dc)::value;
+        hipLaunchKernelGGL(kSdfVoxelsCoop<D>, dim3(paddedGrid(nLogical)), dim3(kBlock), 0, s, m, dims[0], dims[1],
+                           zBegin, list, count, nLogical, xf, o, out);
         return hipGetLastError();
     });
 }
@@ -895,7 +992,8 @@ hipError_t launchSelfIntersections(const MeshDev& m, float beta, uint8_t* out, i
 
 hipError_t launchProjections(const MeshDev& m, const float* pts, const uint32_t* perm, int64_t n,
                              const Xf* ptXf, const Xf* treeXf, float up, float lo, ProjRec* out,
-                             int stackD, hipStream_t s)
+                             int stackD, hipStream_t s, const float* bound, uint32_t* rerun,
+                             uint32_t* rerunCount)
 {
     if (n <= 0) return hipSuccess;
     const int nLogical = (int)((n + kBlock - 1) / kBlock);
@@ -904,18 +1002,20 @@ hipError_t launchProjections(const MeshDev& m, const float* pts, const uint32_t*
     if (treeXf) tx = *treeXf;
     // the shared walk reproduces the reference's record exactly when its early exit cannot pick a
     // non-minimal leaf (loDistLimitSq <= 0) and the tree is not transformed per box
-    const bool coop = traversal() == Traversal::WaveCoop && !(lo > 0) && !treeXf;
+    const bool coopOk = !(lo > 0) && !treeXf;
+    const bool coop = coopOk && (traversal() == Traversal::WaveCoop || rerun);
+    if (rerun && !coop) return hipErrorInvalidValue;
     return dispatchDepth(stackD, [&](auto Dc) {
         constexpr int D = decltype(Dc)::value;
         if (coop)
             hipLaunchKernelGGL(kProjectionsCoop<D>, dim3(paddedGrid(nLogical)), dim3(kBlock), 0, s, m, pts,
-                               perm, n, nLogical, px, ptXf ? 1 : 0, up, out);
+                               perm, n, nLogical, px, ptXf ? 1 : 0, up, bound, rerun, rerunCount, out);
         else if (treeXf)
             hipLaunchKernelGGL((kProjections<true, D>), dim3(paddedGrid(nLogical)), dim3(kBlock), 0, s,
-                               m, pts, perm, n, nLogical, px, ptXf ? 1 : 0, tx, up, lo, out);
+                               m, pts, perm, n, nLogical, px, ptXf ? 1 : 0, tx, up, lo, bound, out);
         else
             hipLaunchKernelGGL((kProjections<false, D>), dim3(paddedGrid(nLogical)), dim3(kBlock), 0, s,
-                               m, pts, perm, n, nLogical, px, ptXf ? 1 : 0, tx, up, lo, out);
+                               m, pts, perm, n, nLogical, px, ptXf ? 1 : 0, tx, up, lo, bound, out);
         return hipGetLastError();
     });
 }

@@ -31,21 +31,33 @@ enum class Traversal : int { PerLane = 0, WaveCoop = 1 };
 void setTraversal(Traversal t);
 Traversal traversal();
 
+// small fill kernel (used instead of hipMemsetAsync so streams stay graph-capture safe)
+hipError_t fillU32(uint32_t* p, int n, uint32_t v, hipStream_t s);
+
 // stack depth template actually used for a tree of the given depth (0 if unsupported)
 int stackDepthFor(int treeDepth);
 
 // z-layer slab [zBegin, zEnd) of the grid, output x-fastest
+// bound (optional): per-voxel cap on maxDistSq (indexed like out); with rerun lists given, voxels
+// whose cap found nothing are appended (slab-relative ids) instead of written
 hipError_t launchSdfGrid(const MeshDev& m, const int32_t dims[3], const Xf& xf, const DistOpts& o,
-                         int32_t zBegin, int32_t zEnd, float* out, int stackD, hipStream_t s);
+                         int32_t zBegin, int32_t zEnd, float* out, int stackD, hipStream_t s,
+                         const float* bound = nullptr, uint32_t* rerun = nullptr, uint32_t* rerunCount = nullptr);
+// the listed slab voxels, searched without a cap
+hipError_t launchSdfVoxels(const MeshDev& m, const int32_t dims[3], const Xf& xf, const DistOpts& o,
+                           int32_t zBegin, const uint32_t* list, int64_t count, float* out, int stackD,
+                           hipStream_t s);
 hipError_t launchFwnGrid(const MeshDev& m, const int32_t dims[3], const Xf& xf, float beta,
                          int32_t zBegin, int32_t zEnd, float* out, int stackD, hipStream_t s);
 // perm may be null (identity order)
 hipError_t launchFwnPoints(const MeshDev& m, const float* pts, const uint32_t* perm, int64_t n,
                            float beta, int32_t skipFace, float* out, int stackD, hipStream_t s);
 hipError_t launchSelfIntersections(const MeshDev& m, float beta, uint8_t* out, int stackD, hipStream_t s);
+// bound (optional): per-query cap on up (indexed by query); rerun lists as for launchSdfGrid
 hipError_t launchProjections(const MeshDev& m, const float* pts, const uint32_t* perm, int64_t n,
                              const Xf* ptXf, const Xf* treeXf, float up, float lo, ProjRec* out,
-                             int stackD, hipStream_t s);
+                             int stackD, hipStream_t s, const float* bound = nullptr,
+                             uint32_t* rerun = nullptr, uint32_t* rerunCount = nullptr);
 
 // spatial ordering of query points: Morton codes over box [lo, hi] then a radix sort.
 // workspace layout is private; query its size first with ws == nullptr.

@@ -1,0 +1,439 @@
+// mi_mlp.hip -- the learned distance bound: a Fourier-feature MLP trained and evaluated on gfx950
+// with hand-written BF16 MFMA kernels (v_mfma_f32_16x16x32_bf16, FP32 accumulation).
+//
+// Network: x^ = (p - centre) / scale; features [x^ (3), sin(pi f_k x^_d), cos(pi f_k x^_d)] for
+// 16 bands f_k = 2^(k/3) (99 dims, zero-padded to 128) -> 4 x (Linear 128 + SiLU) -> Linear 1 ->
+// softplus = distance / scale.  Parameters live in one FP32 vector (master copy, Adam state) with
+// BF16 copies W (out-major) and W^T (in-major) that the MFMA kernels read.
+//
+// Tile shape: a workgroup of 4 waves owns 64 rows; wave w computes rows [16w, 16w+16) of every
+// layer as 8 output tiles of 16x16, K = 128 in 4 MFMA steps of 32.  Activations stay in LDS between
+// layers; each layer's weights are staged once per workgroup into LDS (rows padded to 136 BF16 so
+// the 16-byte fragment reads of 16 rows fall in distinct banks).
+//
+// MFMA 16x16x32 BF16 fragments (lane l): A row l%16, K elements 8*(l/16)..+7; B column l%16, the
+// same K elements; accumulator element j sits at row 4*(l/16)+j, column l%16.
+#include "mi_mlp.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+namespace mi {
+namespace mlp {
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int LDP = kHidden + 8;   // padded LDS row (BF16 elements) for 128-wide rows
+constexpr int LDQ = kTileRows + 8; // padded LDS row for 64-wide (transposed) rows
+constexpr int HP = kHidden + 4;    // padded FP32 row
+
+__device__ __forceinline__ uint16_t f2bf(float x)
+{
+    uint32_t u = __float_as_uint(x);
+    u += 0x7FFFu + ((u >> 16) & 1u);     // round to nearest even
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+__device__ __forceinline__ bf16x8 frag(const uint16_t* p)
+{
+    const uint4 v = *reinterpret_cast<const uint4*>(p);
+    return __builtin_bit_cast(bf16x8, v);
+}
+
+__device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ float sigmoidf(float z) { return 1.f / (1.f + expf(-z)); }
+__device__ __forceinline__ float siluf(float z) { return z / (1.f + expf(-z)); }
+__device__ __forceinline__ float softplusf(float u) { return u > 20.f ? u : log1pf(expf(u)); }
+
+// feature `i` (0..127) of the normalised point x
+__device__ __forceinline__ float feature(int i, const float x[3], const float* piFreq)
+{
+    if (i < 3) return x[i];
+    if (i >= kFeatures) return 0.f;
+    const int j = i - 3, k = j / 6, r = j % 6;
+    const float a = piFreq[k] * x[r % 3];
+    return r < 3 ? sinf(a) : cosf(a);
+}
+
+__device__ __forceinline__ void normalised(const Params& p, const float* pt, float x[3])
+{
+    x[0] = (pt[0] - p.center[0]) * p.invScale;
+    x[1] = (pt[1] - p.center[1]) * p.invScale;
+    x[2] = (pt[2] - p.center[2]) * p.invScale;
+}
+
+// one 64-row tile through the four hidden layers; Act holds the layer input on entry and the
+// last hidden activation on exit
+template <bool TRAIN>
+__device__ __forceinline__ void hiddenLayers(const Params& p, uint16_t* Ws, uint16_t* Act, int64_t row0, int64_t n,
+                                             float* Zs, uint16_t* Hs)
+{
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int lr = lane & 15, lk = lane >> 4;
+    const int r0 = 16 * w;
+    for (int l = 0; l < kLayers; ++l) {
+        __syncthreads();
+        const uint16_t* W = p.Wb + (size_t)l * kHidden * kHidden;
+        for (int it = 0; it < 8; ++it) {          // 128 x 128 BF16 = 2048 chunks of 16 B
+            const int c = it * 256 + tid, row = c >> 4, c8 = (c & 15) * 8;
+            *reinterpret_cast<uint4*>(&Ws[row * LDP + c8]) = *reinterpret_cast<const uint4*>(&W[row * kHidden + c8]);
+        }
+        __syncthreads();
+        bf16x8 a[4];
+#pragma unroll
+        for (int kc = 0; kc < 4; ++kc) a[kc] = frag(&Act[(r0 + lr) * LDP + kc * 32 + lk * 8]);
+        f32x4 acc[8];
+#pragma unroll
+        for (int nt = 0; nt < 8; ++nt) {
+            acc[nt] = f32x4{ 0.f, 0.f, 0.f, 0.f };
+#pragma unroll
+            for (int kc = 0; kc < 4; ++kc) acc[nt] = mfma(a[kc], frag(&Ws[(nt * 16 + lr) * LDP + kc * 32 + lk * 8]), acc[nt]);
+        }
+        const float* bias = p.theta + kOffB + l * kHidden;
+#pragma unroll
+        for (int nt = 0; nt < 8; ++nt) {
+            const int col = nt * 16 + lr;
+            const float bc = bias[col];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int row = r0 + 4 * lk + j;
+                const float z = acc[nt][j] + bc;
+                const uint16_t hb = f2bf(siluf(z));
+                Act[row * LDP + col] = hb;
+                if (TRAIN) {
+                    const int64_t gi = row0 + row;
+                    if (gi < n) {
+                        Zs[((size_t)l * n + gi) * kHidden + col] = z;
+                        Hs[((size_t)l * n + gi) * kHidden + col] = hb;
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// capMargin >= 0: write the capped search radius (pred * (1 + margin))^2 instead of the distance
+template <bool TRAIN>
+__global__ __launch_bounds__(256) void kForward(Params p, const float* __restrict__ pts,
+                                                const uint32_t* __restrict__ perm, int64_t n,
+                                                float* __restrict__ out, float capMargin, float* __restrict__ Zs,
+                                                uint16_t* __restrict__ Hs, float* __restrict__ U)
+{
+    __shared__ __attribute__((aligned(16))) uint16_t Ws[kHidden * LDP];
+    __shared__ __attribute__((aligned(16))) uint16_t Act[kTileRows * LDP];
+    const int tid = threadIdx.x;
+    const int64_t row0 = (int64_t)blockIdx.x * kTileRows;
+    {
+        const int r = tid >> 2, part = tid & 3;
+        const int64_t gi = row0 + r;
+        float x[3] = { 0.f, 0.f, 0.f };
+        if (gi < n) {
+            const int64_t q = perm ? (int64_t)perm[gi] : gi;
+            normalised(p, pts + 3 * q, x);
+        }
+        for (int j = 0; j < 32; ++j) {
+            const int fi = 32 * part + j;
+            Act[r * LDP + fi] = f2bf(feature(fi, x, p.piFreq));
+        }
+    }
+    hiddenLayers<TRAIN>(p, Ws, Act, row0, n, Zs, Hs);
+    // output layer: u = sum_c h4[c] * w5[c] + b5 (four partial sums of 32, combined pairwise)
+    const int r = tid >> 2, part = tid & 3;
+    float s = 0.f;
+    for (int j = 0; j < 32; ++j) {
+        const int c = 32 * part + j;
+        s += bf2f(Act[r * LDP + c]) * bf2f(p.w5b[c]);
+    }
+    s += __shfl_xor(s, 1);
+    s += __shfl_xor(s, 2);
+    const int64_t gi = row0 + r;
+    if (part != 0 || gi >= n) return;
+    const float u = s + p.theta[kOffB5];
+    if (TRAIN) U[gi] = u;
+    if (out) {
+        const int64_t q = perm ? (int64_t)perm[gi] : gi;
+        const float d = softplusf(u) * p.scale;
+        if (capMargin >= 0.f) {
+            const float c = d * (1.f + capMargin);
+            out[q] = c * c;
+        } else {
+            out[q] = d;
+        }
+    }
+}
+
+// loss = 1/(2B) * sum (pred/scale - label/scale)^2 over the batch; per-tile partial gradients
+__global__ __launch_bounds__(256) void kBackward(Params p, const float* __restrict__ pts,
+                                                 const float* __restrict__ labels, int64_t n, float invB,
+                                                 const float* __restrict__ Zs, const uint16_t* __restrict__ Hs,
+                                                 const float* __restrict__ U, float* __restrict__ partial)
+{
+    __shared__ __attribute__((aligned(16))) uint16_t WT[kHidden * LDP];
+    __shared__ __attribute__((aligned(16))) uint16_t dZr[kTileRows * LDP];
+    __shared__ __attribute__((aligned(16))) uint16_t dZT[kHidden * LDQ];
+    __shared__ __attribute__((aligned(16))) uint16_t HT[kHidden * LDQ];
+    __shared__ float dH[kTileRows * HP];
+    __shared__ float duS[kTileRows];
+    __shared__ float red[2][kHidden];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int lr = lane & 15, lk = lane >> 4;
+    const int64_t row0 = (int64_t)blockIdx.x * kTileRows;
+    float* part = partial + (size_t)blockIdx.x * kParamsPadded;
+
+    if (tid < kTileRows) {
+        const int64_t gi = row0 + tid;
+        float du = 0.f;
+        if (gi < n) {
+            const float u = U[gi];
+            const float diff = softplusf(u) - labels[gi] * p.invScale;
+            du = diff * invB * sigmoidf(u);
+        }
+        duS[tid] = du;
+    }
+    __syncthreads();
+    // output layer: dw5[c] = sum_r du[r] h4[r][c], db5 = sum_r du[r], dH4[r][c] = du[r] w5[c]
+    {
+        const int c = tid & 127, half = tid >> 7;
+        float s = 0.f;
+        for (int rr = 0; rr < 32; ++rr) {
+            const int r = half * 32 + rr;
+            const int64_t gi = row0 + r;
+            if (gi < n) s += duS[r] * bf2f(Hs[((size_t)(kLayers - 1) * n + gi) * kHidden + c]);
+        }
+        red[half][c] = s;
+    }
+    for (int e = tid; e < kTileRows * kHidden; e += 256) {
+        const int r = e >> 7, c = e & 127;
+        dH[r * HP + c] = duS[r] * bf2f(p.w5b[c]);
+    }
+    __syncthreads();
+    if (tid < kHidden) part[kOffW5 + tid] = red[0][tid] + red[1][tid];
+    if (tid == 0) {
+        float s = 0.f;
+        for (int r = 0; r < kTileRows; ++r) s += duS[r];
+        part[kOffB5] = s;
+    }
+    for (int l = kLayers - 1; l >= 0; --l) {
+        // dZ = dH * silu'(Z), kept as FP32 in dH (for the bias gradient) and as BF16 in both layouts
+        for (int e = tid; e < kTileRows * kHidden; e += 256) {
+            const int r = e >> 7, c = e & 127;
+            const int64_t gi = row0 + r;
+            float dz = 0.f;
+            if (gi < n) {
+                const float z = Zs[((size_t)l * n + gi) * kHidden + c];
+                const float s = sigmoidf(z);
+                dz = dH[r * HP + c] * (s * (1.f + z * (1.f - s)));
+            }
+            dH[r * HP + c] = dz;
+            const uint16_t b = f2bf(dz);
+            dZr[r * LDP + c] = b;
+            dZT[c * LDQ + r] = b;
+        }
+        // the layer input, transposed: H_{l-1} or (l == 0) the features
+        if (l > 0) {
+            for (int e = tid; e < kTileRows * kHidden; e += 256) {
+                const int r = e >> 7, c = e & 127;
+                const int64_t gi = row0 + r;
+                HT[c * LDQ + r] = gi < n ? Hs[((size_t)(l - 1) * n + gi) * kHidden + c] : (uint16_t)0;
+            }
+        } else {
+            const int r = tid >> 2, q4 = tid & 3;
+            const int64_t gi = row0 + r;
+            float x[3] = { 0.f, 0.f, 0.f };
+            if (gi < n) normalised(p, pts + 3 * gi, x);
+            for (int j = 0; j < 32; ++j) {
+                const int fi = 32 * q4 + j;
+                HT[fi * LDQ + r] = gi < n ? f2bf(feature(fi, x, p.piFreq)) : (uint16_t)0;
+            }
+        }
+        __syncthreads();
+        if (tid < kHidden) {
+            float s = 0.f;
+            for (int r = 0; r < kTileRows; ++r) s += dH[r * HP + tid];
+            part[kOffB + l * kHidden + tid] = s;
+        }
+        // dW_l (128 out x 128 in) = dZ^T (out x rows) . H (rows x in); wave w: out rows [32w, 32w+32)
+        {
+            f32x4 acc[2][8];
+#pragma unroll
+            for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+                for (int nt = 0; nt < 8; ++nt) acc[mt][nt] = f32x4{ 0.f, 0.f, 0.f, 0.f };
+#pragma unroll
+            for (int kc = 0; kc < 2; ++kc) {
+                bf16x8 a0 = frag(&dZT[(32 * w + lr) * LDQ + kc * 32 + lk * 8]);
+                bf16x8 a1 = frag(&dZT[(32 * w + 16 + lr) * LDQ + kc * 32 + lk * 8]);
+#pragma unroll
+                for (int nt = 0; nt < 8; ++nt) {
+                    const bf16x8 b = frag(&HT[(16 * nt + lr) * LDQ + kc * 32 + lk * 8]);
+                    acc[0][nt] = mfma(a0, b, acc[0][nt]);
+                    acc[1][nt] = mfma(a1, b, acc[1][nt]);
+                }
+            }
+            float* dW = part + kOffW + l * kHidden * kHidden;
+#pragma unroll
+            for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+                for (int nt = 0; nt < 8; ++nt)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        dW[(32 * w + 16 * mt + 4 * lk + j) * kHidden + 16 * nt + lr] = acc[mt][nt][j];
+        }
+        if (l == 0) break;
+        // dH_{l-1} (rows x in) = dZ (rows x out) . W_l (out x in); B(k = out, n = in) = W^T[in][out]
+        const uint16_t* Wt = p.WbT + (size_t)l * kHidden * kHidden;
+        for (int it = 0; it < 8; ++it) {
+            const int c = it * 256 + tid, row = c >> 4, c8 = (c & 15) * 8;
+            *reinterpret_cast<uint4*>(&WT[row * LDP + c8]) = *reinterpret_cast<const uint4*>(&Wt[row * kHidden + c8]);
+        }
+        __syncthreads();
+        {
+            f32x4 acc[8];
+            bf16x8 a[4];
+#pragma unroll
+            for (int kc = 0; kc < 4; ++kc) a[kc] = frag(&dZr[(16 * w + lr) * LDP + kc * 32 + lk * 8]);
+#pragma unroll
+            for (int nt = 0; nt < 8; ++nt) {
+                acc[nt] = f32x4{ 0.f, 0.f, 0.f, 0.f };
+#pragma unroll
+                for (int kc = 0; kc < 4; ++kc)
+                    acc[nt] = mfma(a[kc], frag(&WT[(16 * nt + lr) * LDP + kc * 32 + lk * 8]), acc[nt]);
+            }
+            __syncthreads();   // every wave is done reading dH (bias sums) before it is overwritten
+#pragma unroll
+            for (int nt = 0; nt < 8; ++nt)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) dH[(16 * w + 4 * lk + j) * HP + 16 * nt + lr] = acc[nt][j];
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void kReduce(const float* __restrict__ partial, int nBlocks, float* __restrict__ grad)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= kParamsPadded) return;
+    float s = 0.f;
+    for (int b = 0; b < nBlocks; ++b) s += partial[(size_t)b * kParamsPadded + j];
+    grad[j] = s;
+}
+
+// refreshes the BF16 copies (W, W^T, w5) from the FP32 master
+__device__ __forceinline__ void refresh(const Params& p, int j, float v)
+{
+    if (j < kOffB) {
+        const int l = j / (kHidden * kHidden), rem = j % (kHidden * kHidden);
+        const int o = rem / kHidden, i = rem % kHidden;
+        const uint16_t b = f2bf(v);
+        p.WbMut[j] = b;
+        p.WbTMut[(size_t)l * kHidden * kHidden + i * kHidden + o] = b;
+    } else if (j >= kOffW5 && j < kOffW5 + kHidden) {
+        p.w5bMut[j - kOffW5] = f2bf(v);
+    }
+}
+
+__global__ void kAdam(Params p, const float* __restrict__ grad, float* __restrict__ mom, float* __restrict__ vel,
+                      float lr, float b1, float b2, float eps, float c1, float c2)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= kParams) return;
+    const float g = grad[j];
+    const float m = b1 * mom[j] + (1.f - b1) * g;
+    const float v = b2 * vel[j] + (1.f - b2) * g * g;
+    mom[j] = m;
+    vel[j] = v;
+    const float th = p.thetaMut[j] - lr * (m * c1) / (sqrtf(v * c2) + eps);
+    p.thetaMut[j] = th;
+    refresh(p, j, th);
+}
+
+__global__ void kRefreshAll(Params p)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= kParams) return;
+    refresh(p, j, p.theta[j]);
+}
+
+// voxel-centre points of the slab [zBegin, zEnd) in x-fastest order (the network's inputs)
+__global__ void kGridPoints(int dx, int dy, int zBegin, int64_t count, float m0, float m1, float m2, float m3,
+                            float m4, float m5, float m6, float m7, float m8, float b0, float b1, float b2,
+                            float* __restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const int64_t layer = (int64_t)dx * dy;
+    const float z = (float)(zBegin + (int)(i / layer));
+    const int rem = (int)(i % layer);
+    const float x = (float)(rem % dx), y = (float)(rem / dx);
+    out[3 * i] = (m0 * x + m1 * y + m2 * z) + b0;
+    out[3 * i + 1] = (m3 * x + m4 * y + m5 * z) + b1;
+    out[3 * i + 2] = (m6 * x + m7 * y + m8 * z) + b2;
+}
+
+} // namespace
+
+hipError_t forward(const Params& p, const float* pts, const uint32_t* perm, int64_t n, float* out, float capMargin,
+                   hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    const int blocks = (int)((n + kTileRows - 1) / kTileRows);
+    hipLaunchKernelGGL(kForward<false>, dim3(blocks), dim3(256), 0, s, p, pts, perm, n, out, capMargin, nullptr,
+                       nullptr, nullptr);
+    return hipGetLastError();
+}
+
+hipError_t gradient(const Params& p, const float* pts, const float* labels, int64_t n, const TrainBuffers& tb,
+                    float* grad, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    const int blocks = (int)((n + kTileRows - 1) / kTileRows);
+    hipLaunchKernelGGL(kForward<true>, dim3(blocks), dim3(256), 0, s, p, pts, nullptr, n, nullptr, -1.f, tb.Z, tb.H,
+                       tb.U);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(kBackward, dim3(blocks), dim3(256), 0, s, p, pts, labels, n, 1.f / (float)n, tb.Z, tb.H,
+                       tb.U, tb.partial);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(kReduce, dim3((kParamsPadded + 255) / 256), dim3(256), 0, s, tb.partial, blocks, grad);
+    return hipGetLastError();
+}
+
+hipError_t adamStep(const Params& p, const float* grad, float* mom, float* vel, int step, float lr, hipStream_t s)
+{
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    const float c1 = (float)(1.0 / (1.0 - std::pow(0.9, step)));
+    const float c2 = (float)(1.0 / (1.0 - std::pow(0.999, step)));
+    hipLaunchKernelGGL(kAdam, dim3((kParams + 255) / 256), dim3(256), 0, s, p, grad, mom, vel, lr, b1, b2, eps, c1, c2);
+    return hipGetLastError();
+}
+
+hipError_t refreshCopies(const Params& p, hipStream_t s)
+{
+    hipLaunchKernelGGL(kRefreshAll, dim3((kParams + 255) / 256), dim3(256), 0, s, p);
+    return hipGetLastError();
+}
+
+hipError_t gridPoints(const int32_t dims[3], const float xf[12], int32_t zBegin, int32_t zEnd, float* out,
+                      hipStream_t s)
+{
+    const int64_t count = (int64_t)dims[0] * dims[1] * (zEnd - zBegin);
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(kGridPoints, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, dims[0], dims[1], zBegin,
+                       count, xf[0], xf[1], xf[2], xf[3], xf[4], xf[5], xf[6], xf[7], xf[8], xf[9], xf[10], xf[11],
+                       out);
+    return hipGetLastError();
+}
+
+} // namespace mlp
+} // namespace mi
