@@ -102,6 +102,89 @@ def learned_leg(M, torch, np, synth, v, f, mesh, fwn, proj, dims, xf, opts, d_pt
 STEPS_TRAIN = [2000]
 
 
+def batched_main(args):
+    """Uneven batch: 64 bumpy icospheres with split level uniform in 4..8, each with a signed-distance
+    grid of side in {32, 64, 96, 128, 192} and 2^18..2^21 projection queries.  Compares one
+    autotuned persistent launch with per-item launches (wave-cooperative and per-lane forms)."""
+    import numpy as np
+    import torch
+
+    import meshlib_amd as M
+    from meshlib_amd import synth
+
+    torch.cuda.set_device(0)
+    rng = np.random.Generator(np.random.PCG64(4))
+    n_items = args.items
+    levels = rng.integers(4, 9, size=n_items)
+    sides = rng.choice([32, 64, 96, 128, 192], size=n_items)
+    nqs = rng.integers(1 << 18, (1 << 21) + 1, size=n_items)
+    t0 = time.time()
+    items, keep, per_item = [], [], []
+    tot_vox = tot_pts = 0
+    for i in range(n_items):
+        v, f = synth.bumpy_icosphere(int(levels[i]), 32, 1000 + i)
+        mesh = M.Mesh(v, f)
+        dims, xf, vox = synth.grid_for_mesh(v, (int(sides[i]),) * 3)
+        pts = synth.projection_queries(v, f, int(nqs[i]), 2000 + i)
+        dp = torch.from_numpy(pts).cuda()
+        ds = torch.empty(int(np.prod(dims)), dtype=torch.float32, device="cuda")
+        dr = torch.empty(int(nqs[i]) * 8, dtype=torch.int32, device="cuda")
+        items.append({"mesh": mesh, "dims": dims, "xf": xf, "d_sdf": ds.data_ptr(), "d_pts": dp.data_ptr(),
+                      "n": int(nqs[i]), "d_proj": dr.data_ptr()})
+        fwn = M.FastWindingNumber(mesh)
+        pr = M.PointsToMeshProjector()
+        pr.updateMeshData(mesh)
+        per_item.append((fwn, pr, dims, xf, ds, dp, dr))
+        keep += [dp, ds, dr]
+        tot_vox += int(np.prod(dims))
+        tot_pts += int(nqs[i])
+    log(f"batched inputs {time.time() - t0:.1f}s: {n_items} meshes, {tot_vox} voxels, {tot_pts} points")
+    opts = M.DistanceToMeshOptions()
+    stream = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+
+    def timed(fn):
+        ev[0].record(stream)
+        fn()
+        ev[1].record(stream)
+        torch.cuda.synchronize()
+        return ev[0].elapsed_time(ev[1])
+
+    def per_item_run():
+        for fwn, pr, dims, xf, ds, dp, dr in per_item:
+            fwn.sdf_grid_device(dims, xf, opts, ds.data_ptr())
+            pr.find_projections_device(dp.data_ptr(), dp.shape[0], dr.data_ptr())
+
+    M.set_traversal("wave_coop")
+    per_item_run()
+    torch.cuda.synchronize()
+    t_items = timed(per_item_run)
+    ref = [(ds.clone(), dr.clone()) for (_, _, _, _, ds, _, dr) in per_item]
+    batch = M.Batch()
+    t_tune = batch.autotune(items, opts)
+    cfg = batch.config()
+    t_batch = min(timed(lambda: batch.run(items, opts)) for _ in range(args.steps))
+    same = all(torch.equal(ds.view(torch.int32), r[0].view(torch.int32)) and torch.equal(dr, r[1])
+               for (_, _, _, _, ds, _, dr), r in zip(per_item, ref))
+    t_plain = None
+    if not args.no_plain:
+        M.set_traversal("per_lane")
+        t_plain = timed(per_item_run)
+        M.set_traversal("wave_coop")
+    out = {"metric": "signed-dist voxels/s & projected pts/s (batched uneven workload)",
+           "value": (tot_vox + tot_pts) / (t_batch / 1e3), "unit": "queries/s (voxels + projected points)",
+           "n_gpus": 1, "steps": args.steps, "ms_per_step": t_batch, "higher_is_better": True,
+           "data": "synthetic", "dtype": "f32+f64",
+           "config": {"workload": f"{n_items} bumpy icospheres, levels 4..8, grids 32..192, 2^18..2^21 points each",
+                      "voxels": tot_vox, "points": tot_pts},
+           "batched_ms": t_batch, "autotune": {"blocks_per_cu": cfg[0], "xcd_split": cfg[1], "best_ms": t_tune},
+           "per_item_coop_ms": t_items, "per_item_per_lane_ms": t_plain,
+           "speedup_vs_per_item_coop": t_items / t_batch,
+           "speedup_vs_per_item_per_lane": (t_plain / t_batch) if t_plain else None,
+           "identical_to_per_item": bool(same)}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -115,8 +198,13 @@ def main():
                     help="skip the one-step comparison run with the per-lane (reference-form) traversal")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-bound leg")
     ap.add_argument("--train-steps", type=int, default=2000)
+    ap.add_argument("--workload", default="single", choices=["single", "batched"],
+                    help="single: the k=8 mesh step (default); batched: the uneven multi-mesh batch")
+    ap.add_argument("--items", type=int, default=64, help="meshes in the batched workload")
     args = ap.parse_args()
     STEPS_TRAIN[0] = args.train_steps
+    if args.workload == "batched" and args.gpus == 1:
+        return batched_main(args)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: this build has no multi-GPU path (asked for --gpus "
               f"{args.gpus}); the path shards by z-slabs / query ranges with no exchange, see DESIGN.md",

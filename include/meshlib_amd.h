@@ -188,6 +188,37 @@ int mi_sdf_grid_capped_device(mi_mesh* m, const int32_t dims[3], const float xf[
                               const float* d_capSq, int32_t zBegin, int32_t zEnd, float* d_res,
                               int64_t* reruns, void* stream);
 
+/* ---------------------------------------------------------------- batched persistent launches
+ * No counterpart in the reference (it launches one kernel per call and per chunk,
+ * source/MRCuda/MRCudaFastWindingNumber.cpp:260-280, MRCudaPointsToMeshProjector.cpp:219-227):
+ * one persistent launch runs many items -- each a signed-distance grid (calcFromGridWithDistances)
+ * and/or a point set (findProjections) on its own mesh -- with work units of 64 voxels or 64
+ * queries taken from per-XCD ticket counters.  Outputs equal the per-item calls. */
+typedef struct mi_batch_item {
+    mi_mesh* mesh;
+    int32_t dims[3];          /* grid dimensions; any 0 = no grid */
+    float xf[12];             /* gridToMeshXf, row-major A then b */
+    float* d_sdf;             /* dims.x*dims.y*dims.z floats, x-fastest */
+    const float* d_pts;       /* numPts float3 points; 0 = no projections */
+    int64_t numPts;
+    mi_projection* d_proj;    /* numPts records */
+} mi_batch_item;
+
+typedef struct mi_batch mi_batch;
+int  mi_batch_create(mi_batch** out);
+void mi_batch_destroy(mi_batch* b);
+/* opts apply to every grid (minDistSq must be <= 0); upDistLimitSq to every point set
+ * (loDistLimitSq = 0) */
+int  mi_batch_run_device(mi_batch* b, const mi_batch_item* items, int32_t count, const mi_dist_options* opts,
+                         float upDistLimitSq, void* stream);
+/* the project's autotuner (role of runProfiling, source/MRCuda/MRCudaProfiling.cuh:22-79): times the
+ * batch over resident workgroups per CU {2..6} x XCD partitioning {off, on} and keeps the fastest
+ * for later runs; *bestMs = its time (the outputs are computed as a side effect) */
+int  mi_batch_autotune(mi_batch* b, const mi_batch_item* items, int32_t count, const mi_dist_options* opts,
+                       float upDistLimitSq, void* stream, float* bestMs);
+int  mi_batch_set_config(mi_batch* b, int32_t blocksPerCU, int32_t xcdSplit);
+int  mi_batch_get_config(mi_batch* b, int32_t* blocksPerCU, int32_t* xcdSplit);
+
 /* ---------------------------------------------------------------- learned distance bound
  * No counterpart in the reference: a Fourier-feature MLP (99 features -> 4 x 128 SiLU -> softplus)
  * predicts each query's unsigned distance; the prediction caps the search radius at

@@ -115,6 +115,12 @@ def lib():
         "mi_proj_find_projections_bounded_device": (C.c_int, [P, P, I64, P, P, P, F, F, P, P]),
         "mi_proj_find_projections_capped_device": (C.c_int, [P, P, I64, P, F, P, C.POINTER(I64), P]),
         "mi_sdf_grid_capped_device": (C.c_int, [P, P, P, P, P, I32, I32, P, C.POINTER(I64), P]),
+        "mi_batch_create": (C.c_int, [C.POINTER(P)]),
+        "mi_batch_destroy": (None, [P]),
+        "mi_batch_run_device": (C.c_int, [P, P, I32, P, F, P]),
+        "mi_batch_autotune": (C.c_int, [P, P, I32, P, F, P, C.POINTER(F)]),
+        "mi_batch_set_config": (C.c_int, [P, I32, I32]),
+        "mi_batch_get_config": (C.c_int, [P, C.POINTER(I32), C.POINTER(I32)]),
         "mi_mlp_create": (C.c_int, [P, F, C.c_uint64, C.POINTER(P)]),
         "mi_mlp_destroy": (None, [P]),
         "mi_mlp_num_params": (C.c_int, []),
@@ -260,6 +266,72 @@ class Mesh:
 
     def depth(self) -> int:
         return int(lib().mi_mesh_depth(self._h))
+
+
+class _BatchItem(C.Structure):
+    _fields_ = [("mesh", C.c_void_p), ("dims", C.c_int32 * 3), ("xf", C.c_float * 12), ("d_sdf", C.c_void_p),
+                ("d_pts", C.c_void_p), ("numPts", C.c_int64), ("d_proj", C.c_void_p)]
+
+
+class Batch:
+    """One persistent launch over many (mesh, grid, point set) items of uneven size, with the
+    project's autotuner for resident workgroups per CU and per-XCD work partitioning."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        _check(lib().mi_batch_create(C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                lib().mi_batch_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    @staticmethod
+    def _items(items):
+        arr = (_BatchItem * len(items))()
+        for i, it in enumerate(items):
+            arr[i].mesh = it["mesh"].handle
+            d = it.get("dims")
+            if d is None:
+                d = (0, 0, 0)
+            for k in range(3):
+                arr[i].dims[k] = int(d[k])
+            x = _xf(it["xf"]) if it.get("xf") is not None else np.zeros(12, np.float32)
+            for k in range(12):
+                arr[i].xf[k] = float(x[k])
+            arr[i].d_sdf = it.get("d_sdf") or None
+            arr[i].d_pts = it.get("d_pts") or None
+            arr[i].numPts = int(it.get("n", 0))
+            arr[i].d_proj = it.get("d_proj") or None
+        return arr
+
+    def run(self, items, options: DistanceToMeshOptions = None, upDistLimitSq: float = FLT_MAX, stream: int = 0):
+        arr = self._items(items)
+        o = (options or DistanceToMeshOptions())._c()
+        _check(lib().mi_batch_run_device(self._h, arr, len(items), C.byref(o), upDistLimitSq,
+                                         C.c_void_p(stream or None)))
+
+    def autotune(self, items, options: DistanceToMeshOptions = None, upDistLimitSq: float = FLT_MAX,
+                 stream: int = 0) -> float:
+        arr = self._items(items)
+        o = (options or DistanceToMeshOptions())._c()
+        ms = C.c_float(0)
+        _check(lib().mi_batch_autotune(self._h, arr, len(items), C.byref(o), upDistLimitSq,
+                                       C.c_void_p(stream or None), C.byref(ms)))
+        return float(ms.value)
+
+    def config(self):
+        a, b = C.c_int32(0), C.c_int32(0)
+        _check(lib().mi_batch_get_config(self._h, C.byref(a), C.byref(b)))
+        return int(a.value), int(b.value)
+
+    def set_config(self, blocks_per_cu: int, xcd_split: int):
+        _check(lib().mi_batch_set_config(self._h, blocks_per_cu, xcd_split))
 
 
 class DistanceMLP:

@@ -731,3 +731,188 @@ size_t mi_proj_heap_bytes(size_t numProjections)
 }
 
 } // extern "C"
+
+// ---------------------------------------------------------------- batched persistent launches
+
+struct mi_batch {
+    int device = 0;
+    int numCUs = 0;
+    int blocksPerCU = 4;
+    int xcdSplit = 1;
+    DevBuf items;        // BatchItem[]
+    DevBuf tickets;      // 8 x u64
+    DevBuf sortWs;       // per-item Morton sort workspaces
+    std::vector<mi::BatchItem> host;
+    std::mutex mu;
+};
+
+namespace {
+
+// builds the device descriptors (sorting every item's points) and returns the unit count
+int prepareBatch(mi_batch* b, const mi_batch_item* items, int32_t count, hipStream_t s, int64_t* total, int* stackD)
+{
+    *total = 0;
+    *stackD = 8;
+    b->host.assign((size_t)count, mi::BatchItem{});
+    size_t wsTotal = 0;
+    for (int32_t i = 0; i < count; ++i) {
+        const mi_batch_item& it = items[i];
+        if (!it.mesh) return fail(MI_E_INVALID, "batch item without a mesh");
+        if (it.numPts < 0 || it.numPts > INT32_MAX || (it.numPts > 0 && (!it.d_pts || !it.d_proj)))
+            return fail(MI_E_INVALID, "invalid batch item points");
+        const int64_t nvox = (int64_t)it.dims[0] * it.dims[1] * it.dims[2];
+        if (it.dims[0] < 0 || it.dims[1] < 0 || it.dims[2] < 0 || (nvox > 0 && !it.d_sdf))
+            return fail(MI_E_INVALID, "invalid batch item grid");
+        if (it.mesh->device != b->device) return fail(MI_E_INVALID, "batch items must live on the batch's device");
+        wsTotal += align256(mi::sortWorkspaceBytes(it.numPts));
+        *stackD = std::max(*stackD, it.mesh->stackD);
+    }
+    if (wsTotal && b->sortWs.bytes < wsTotal) MI_HIP(b->sortWs.alloc(wsTotal));
+    size_t off = 0;
+    int64_t unit = 0;
+    for (int32_t i = 0; i < count; ++i) {
+        const mi_batch_item& it = items[i];
+        mi::BatchItem& d = b->host[i];
+        d.m = it.mesh->dev;
+        d.dx = it.dims[0]; d.dy = it.dims[1]; d.dz = it.dims[2];
+        d.nbx = (d.dx + 3) / 4; d.nby = (d.dy + 3) / 4; d.nbz = (d.dz + 3) / 4;
+        std::memcpy(d.xf.m, it.xf, sizeof(d.xf.m));
+        d.sdf = it.d_sdf;
+        d.gridUnits = (int64_t)d.dx * d.dy * d.dz > 0 ? (int64_t)d.nbx * d.nby * d.nbz : 0;
+        d.pts = it.d_pts;
+        d.npts = it.numPts;
+        d.proj = reinterpret_cast<mi::ProjRec*>(it.d_proj);
+        d.projUnits = (it.numPts + 63) / 64;
+        d.perm = nullptr;
+        if (it.numPts > 0) {
+            const size_t wsb = align256(mi::sortWorkspaceBytes(it.numPts));
+            const uint32_t* perm = nullptr;
+            MI_HIP(mi::sortPointsMorton(it.d_pts, it.numPts, it.mesh->sortLo, it.mesh->sortHi,
+                                        b->sortWs.as<char>() + off, wsb, &perm, s));
+            d.perm = perm;
+            off += wsb;
+        }
+        d.unitBegin = unit;
+        unit += d.gridUnits + d.projUnits;
+    }
+    if ((size_t)count * sizeof(mi::BatchItem) > b->items.bytes)
+        MI_HIP(b->items.alloc((size_t)count * sizeof(mi::BatchItem)));
+    // pageable source: the copy is complete when hipMemcpyAsync returns
+    MI_HIP(hipMemcpyAsync(b->items.p, b->host.data(), (size_t)count * sizeof(mi::BatchItem), hipMemcpyHostToDevice, s));
+    *total = unit;
+    return MI_OK;
+}
+
+int launchPrepared(mi_batch* b, int32_t count, int64_t total, int stackD, const mi_dist_options* opts, float up,
+                   int blocksPerCU, int xcdSplit, hipStream_t s)
+{
+    mi::DistOpts o{ opts->minDistSq, opts->maxDistSq, opts->nullOutsideMinMax, opts->windingNumberThreshold,
+                    opts->windingNumberBeta };
+    MI_HIP(mi::fillU32(b->tickets.as<uint32_t>(), 16, 0u, s));
+    MI_HIP(mi::launchBatch(b->items.as<mi::BatchItem>(), count, total, o, up,
+                           b->tickets.as<unsigned long long>(), xcdSplit, b->numCUs * blocksPerCU, stackD, s));
+    return MI_OK;
+}
+
+int checkBatch(mi_batch* b, const mi_batch_item* items, int32_t count, const mi_dist_options* opts)
+{
+    if (!b || count < 0 || (count > 0 && !items) || !opts) return fail(MI_E_INVALID, "invalid batch arguments");
+    if (opts->minDistSq > 0) return fail(MI_E_INVALID, "batched signed distance needs minDistSq <= 0");
+    return MI_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int mi_batch_create(mi_batch** out)
+{
+    if (!out) return fail(MI_E_INVALID, "out is null");
+    *out = nullptr;
+    mi_batch* b = new (std::nothrow) mi_batch();
+    if (!b) return fail(MI_E_NOMEM, "out of host memory");
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&b->device) != hipSuccess || hipGetDeviceProperties(&prop, b->device) != hipSuccess ||
+        b->tickets.alloc(16 * sizeof(uint32_t)) != hipSuccess) {
+        delete b;
+        return fail(MI_E_HIP, "AMD GPU error: cannot set up the batch");
+    }
+    b->numCUs = prop.multiProcessorCount;
+    *out = b;
+    return MI_OK;
+}
+
+void mi_batch_destroy(mi_batch* b) { delete b; }
+
+int mi_batch_set_config(mi_batch* b, int32_t blocksPerCU, int32_t xcdSplit)
+{
+    if (!b || blocksPerCU < 1 || blocksPerCU > 16) return fail(MI_E_INVALID, "invalid batch config");
+    b->blocksPerCU = blocksPerCU;
+    b->xcdSplit = xcdSplit ? 1 : 0;
+    return MI_OK;
+}
+
+int mi_batch_get_config(mi_batch* b, int32_t* blocksPerCU, int32_t* xcdSplit)
+{
+    if (!b) return fail(MI_E_INVALID, "batch is null");
+    if (blocksPerCU) *blocksPerCU = b->blocksPerCU;
+    if (xcdSplit) *xcdSplit = b->xcdSplit;
+    return MI_OK;
+}
+
+int mi_batch_run_device(mi_batch* b, const mi_batch_item* items, int32_t count, const mi_dist_options* opts,
+                        float upDistLimitSq, void* stream)
+{
+    if (int rc = checkBatch(b, items, count, opts)) return rc;
+    std::lock_guard<std::mutex> lk(b->mu);
+    MI_HIP(hipSetDevice(b->device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    int64_t total = 0;
+    int stackD = 8;
+    if (int rc = prepareBatch(b, items, count, s, &total, &stackD)) return rc;
+    return launchPrepared(b, count, total, stackD, opts, upDistLimitSq, b->blocksPerCU, b->xcdSplit, s);
+}
+
+int mi_batch_autotune(mi_batch* b, const mi_batch_item* items, int32_t count, const mi_dist_options* opts,
+                      float upDistLimitSq, void* stream, float* bestMs)
+{
+    if (int rc = checkBatch(b, items, count, opts)) return rc;
+    std::lock_guard<std::mutex> lk(b->mu);
+    MI_HIP(hipSetDevice(b->device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    int64_t total = 0;
+    int stackD = 8;
+    if (int rc = prepareBatch(b, items, count, s, &total, &stackD)) return rc;
+    hipEvent_t e0, e1;
+    MI_HIP(hipEventCreate(&e0));
+    MI_HIP(hipEventCreate(&e1));
+    float best = -1.f;
+    int bestBlocks = b->blocksPerCU, bestSplit = b->xcdSplit;
+    const int blocksCand[] = { 2, 3, 4, 5, 6 };
+    int rc = MI_OK;
+    // warm-up (caches, code object load)
+    rc = launchPrepared(b, count, total, stackD, opts, upDistLimitSq, b->blocksPerCU, b->xcdSplit, s);
+    for (int split = 0; split < 2 && rc == MI_OK; ++split) {
+        for (int bpc : blocksCand) {
+            if (hipEventRecord(e0, s) != hipSuccess) { rc = fail(MI_E_HIP, "AMD GPU error: event"); break; }
+            rc = launchPrepared(b, count, total, stackD, opts, upDistLimitSq, bpc, split, s);
+            if (rc != MI_OK) break;
+            float ms = 0.f;
+            if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+                rc = fail(MI_E_HIP, "AMD GPU error: timing the batch");
+                break;
+            }
+            if (best < 0 || ms < best) { best = ms; bestBlocks = bpc; bestSplit = split; }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc != MI_OK) return rc;
+    b->blocksPerCU = bestBlocks;
+    b->xcdSplit = bestSplit;
+    if (bestMs) *bestMs = best;
+    return MI_OK;
+}
+
+} // extern "C"

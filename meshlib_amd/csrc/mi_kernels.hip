@@ -553,9 +553,11 @@ __device__ __forceinline__ float coopWinding(const MeshDev& m, float qx, float q
 // lexicographically, which makes the record bit-identical to the reference for any visit order.
 // Valid when loDistLimitSq <= 0 (the reference's early exit then only fires at distSq == 0,
 // which is the minimum) and for trees of depth <= 32.
+// No per-lane key is stored on the stack: a pending entry is always the sibling of the current
+// path's node at the entry's depth d, so its key is the current key cut below d with bit d flipped.
 template <int D>
 __device__ __forceinline__ void coopProject(const MeshDev& m, float px, float py, float pz, float up, bool valid,
-                                            float4* stk, uint32_t* keys, BestHit& res)
+                                            float4* stk, BestHit& res)
 {
     res.distSq = valid ? up : -1.f;
     res.face = -1;
@@ -563,7 +565,6 @@ __device__ __forceinline__ void coopProject(const MeshDev& m, float px, float py
     res.a = res.b = 0;
     uint32_t bestKey = 0;
     if (m.numFaces <= 0) return;
-    const int lane = threadIdx.x & 63;
     const double pd[3] = { px, py, pz };
     int sp = 0;
     int cur = m.rootRef, curDepth = 0;
@@ -577,9 +578,10 @@ __device__ __forceinline__ void coopProject(const MeshDev& m, float px, float py
             --sp;
             const float4 e0 = stk[2 * sp], e1 = stk[2 * sp + 1];
             cur = __float_as_int(e0.x);
-            curDepth = __float_as_int(e1.w);
+            curDepth = __float_as_int(e1.w);          // >= 1
             curd = boxDistSq(e0.y, e0.z, e0.w, e1.x, e1.y, e1.z, px, py, pz);
-            curKey = keys[sp * 64 + lane];
+            const uint32_t bit = 1u << (32 - curDepth);
+            curKey = (curKey & ~(bit - 1u)) ^ bit;
         }
         have = false;
         const bool act = curd < res.distSq || (curd == res.distSq && curKey < bestKey);
@@ -628,7 +630,6 @@ __device__ __forceinline__ void coopProject(const MeshDev& m, float px, float py
             if (b2) {
                 stk[2 * sp] = make_float4(n3.y, n1.z, n1.w, n2.x);
                 stk[2 * sp + 1] = make_float4(n2.y, n2.z, n2.w, __int_as_float(childDepth));
-                keys[sp * 64 + lane] = kR;
                 ++sp;
             }
             cur = __float_as_int(n3.x); curd = d1; curKey = kL;
@@ -636,7 +637,6 @@ __device__ __forceinline__ void coopProject(const MeshDev& m, float px, float py
             if (b1) {
                 stk[2 * sp] = make_float4(n3.x, n0.x, n0.y, n0.z);
                 stk[2 * sp + 1] = make_float4(n0.w, n1.x, n1.y, __int_as_float(childDepth));
-                keys[sp * 64 + lane] = kL;
                 ++sp;
             }
             cur = __float_as_int(n3.y); curd = d2; curKey = kR;
@@ -673,7 +673,6 @@ __global__ __launch_bounds__(kBlock) void kProjectionsCoop(MeshDev m, const floa
                                                            ProjRec* __restrict__ out)
 {
     __shared__ float4 stkMem[kWaves][2 * D];
-    __shared__ uint32_t keyMem[kWaves][D * 64];
     const int lb = logicalBlock(nLogical);
     if (lb >= nLogical) return;
     const int64_t i = (int64_t)lb * kBlock + threadIdx.x;
@@ -689,7 +688,7 @@ __global__ __launch_bounds__(kBlock) void kProjectionsCoop(MeshDev m, const floa
     }
     const int w = threadIdx.x >> 6;
     BestHit h;
-    coopProject<D>(m, px, py, pz, lim, valid, stkMem[w], keyMem[w], h);
+    coopProject<D>(m, px, py, pz, lim, valid, stkMem[w], h);
     const bool again = rerun && valid && h.face < 0 && lim < up;
     if (rerun) appendWave(again, (uint32_t)q, rerun, rerunCount);
     if (!valid || again) return;
@@ -825,6 +824,68 @@ __global__ __launch_bounds__(kBlock) void kSelfIntersectionsCoop(MeshDev m, int 
     if (valid) out[L.face] = (wn < 0 || wn > 1) ? 1 : 0;
 }
 
+// ---------------------------------------------------------------- persistent batched kernel
+//
+// One launch over many (mesh, grid, point set) items of uneven size.  Work units are one voxel
+// brick (64 voxels) or one chunk of 64 Morton-sorted queries; each wave takes units from a ticket
+// counter and runs the wave-cooperative traversal of that unit's mesh.  With xcdSplit the unit
+// range is cut into 8 contiguous parts, one per XCD (blocks are dealt to XCDs round-robin), and a
+// wave whose part is drained moves on to the next part: spatially adjacent units share an L2,
+// and the tail is balanced.  Every wave exits once it has seen every part drained.
+template <int D>
+__global__ __launch_bounds__(kBlock) void kBatch(const BatchItem* __restrict__ items, int nItems, int64_t total,
+                                                 DistOpts o, float up, unsigned long long* __restrict__ tickets,
+                                                 int xcdSplit)
+{
+    __shared__ float4 stkMem[kWaves][2 * D];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int parts = xcdSplit ? kXcds : 1;
+    int part = xcdSplit ? (int)(blockIdx.x % kXcds) : 0;
+    int drained = 0;
+    for (;;) {
+        const int64_t pBegin = total * part / parts, pEnd = total * (part + 1) / parts;
+        unsigned long long t = 0;
+        if (lane == 0) t = atomicAdd(&tickets[part], 1ull);
+        const uint32_t tlo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)t, 0);
+        const uint32_t thi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(t >> 32), 0);
+        const int64_t u = pBegin + (int64_t)(((uint64_t)thi << 32) | tlo);
+        if (u >= pEnd) {
+            if (++drained >= parts) break;
+            part = (part + 1) % parts;
+            continue;
+        }
+        int lo = 0, hi = nItems - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (items[mid].unitBegin <= u) lo = mid; else hi = mid - 1;
+        }
+        const BatchItem& it = items[uniform(lo)];
+        const int64_t lu = u - it.unitBegin;
+        if (lu < it.gridUnits) {
+            int x, y, z;
+            const bool valid = brickVoxel((int)lu, lane, it.nbx, it.nby, it.nbz, it.dx, it.dy, 0, it.dz, x, y, z);
+            float px = (float)x, py = (float)y, pz = (float)z;
+            xfApply(it.xf, px, py, pz);
+            const float resSq = coopDistance<D>(it.m, px, py, pz, valid ? o.maxDistSq : -1.f, stkMem[w]);
+            const size_t idx = valid ? ((size_t)z * it.dy + y) * it.dx + x : 0;
+            sdfStore(it.m, px, py, pz, valid, resSq, o, reinterpret_cast<uint4*>(stkMem[w]), it.sdf, idx);
+        } else {
+            const int64_t i = (lu - it.gridUnits) * 64 + lane;
+            const bool valid = i < it.npts;
+            const int64_t q = valid ? (it.perm ? (int64_t)it.perm[i] : i) : 0;
+            float px = 0, py = 0, pz = 0;
+            if (valid) { px = it.pts[3 * q]; py = it.pts[3 * q + 1]; pz = it.pts[3 * q + 2]; }
+            BestHit h;
+            coopProject<D>(it.m, px, py, pz, up, valid, stkMem[w], h);
+            if (valid) {
+                float4* op = reinterpret_cast<float4*>(it.proj + q);
+                op[0] = make_float4(__int_as_float(h.face), h.px, h.py, h.pz);
+                op[1] = make_float4(__int_as_float(-1), h.a, h.b, h.distSq);
+            }
+        }
+    }
+}
+
 __device__ __forceinline__ uint32_t spread10(uint32_t v)
 {
     v &= 0x3ffu;
@@ -869,6 +930,19 @@ __global__ void kFillU32(uint32_t* p, int n, uint32_t v)
     if (i < n) p[i] = v;
 }
 } // namespace
+
+hipError_t launchBatch(const BatchItem* items, int nItems, int64_t totalUnits, const DistOpts& o, float up,
+                       unsigned long long* tickets, int xcdSplit, int blocks, int stackD, hipStream_t s)
+{
+    if (totalUnits <= 0 || nItems <= 0) return hipSuccess;
+    if (o.minDistSq > 0 || blocks <= 0) return hipErrorInvalidValue;
+    return dispatchDepth(stackD, [&](auto Dc) {
+        constexpr int D = decltype(Dc)::value;
+        hipLaunchKernelGGL(kBatch<D>, dim3(blocks), dim3(kBlock), 0, s, items, nItems, totalUnits, o, up, tickets,
+                           xcdSplit);
+        return hipGetLastError();
+    });
+}
 
 hipError_t fillU32(uint32_t* p, int n, uint32_t v, hipStream_t s)
 {

@@ -31,6 +31,25 @@ enum class Traversal : int { PerLane = 0, WaveCoop = 1 };
 void setTraversal(Traversal t);
 Traversal traversal();
 
+// one item of a batched launch: a signed-distance grid (gridUnits bricks of 4x4x4 voxels) and/or a
+// Morton-sorted point set (projUnits chunks of 64), on one mesh
+struct BatchItem {
+    MeshDev m;
+    int32_t dx, dy, dz, nbx, nby, nbz;
+    Xf xf;
+    float* sdf;
+    const float* pts;
+    const uint32_t* perm;
+    int64_t npts;
+    ProjRec* proj;
+    int64_t gridUnits, projUnits, unitBegin;
+};
+
+// persistent launch over items[0..nItems) (device array); tickets = 8 zeroed 64-bit counters;
+// blocks = number of 256-thread workgroups to keep resident
+hipError_t launchBatch(const BatchItem* items, int nItems, int64_t totalUnits, const DistOpts& o, float up,
+                       unsigned long long* tickets, int xcdSplit, int blocks, int stackD, hipStream_t s);
+
 // small fill kernel (used instead of hipMemsetAsync so streams stay graph-capture safe)
 hipError_t fillU32(uint32_t* p, int n, uint32_t v, hipStream_t s);
 
