@@ -31,6 +31,31 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+# HBM traffic of one kSdfGridCoop launch on the default workload, from two rocprofv3 --pmc passes
+# over `bench.py --steps 1 --warmup 0 --no-cpu --no-learned --no-plain --copy-calibration`
+# (profiles/sdf_grid_coop_k8_pmc.md): FETCH_SIZE and WRITE_SIZE in KiB per dispatch, each scaled
+# by bytes / counter reading of the 1 GiB device copy in the same run.
+PMC_SDF_KIB = {"fetch": 20467738.0, "write": 152168.0}
+PMC_COPY_KIB = {"fetch": 524318.0, "write": 1048580.0}
+PMC_SDF_L2_HIT = 602191000.0 / (602191000.0 + 182081000.0)    # TCC_HIT_sum / (HIT + MISS)
+COPY_BYTES = 1 << 30
+
+
+def calibration_copy(torch):
+    a = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    del a, b
+
+
+def pmc_traffic_bytes():
+    """HBM bytes per signed-distance launch, measured (fetch and write scaled separately)."""
+    kib = COPY_BYTES / 1024.0
+    return (PMC_SDF_KIB["fetch"] * kib / PMC_COPY_KIB["fetch"] + PMC_SDF_KIB["write"] * kib / PMC_COPY_KIB["write"]) * 1024.0
+
+
 TRAIN_N, TRAIN_BATCH, TRAIN_LR, TRAIN_SEED = 1 << 20, 8192, 1e-3, 11
 MARGINS = (0.05, 0.1, 0.25)
 # FLOP per sample: forward 2*(99*128 + 3*128*128 + 128); backward 2 GEMMs per layer (dW, dH)
@@ -185,6 +210,84 @@ def batched_main(args):
     print(json.dumps(out), flush=True)
 
 
+def offset_main(args):
+    """Offset-scale volume: bumpy icosphere split 9 times (5,242,880 tris, 64 bumps), 512^3 signed
+    distances (FWN beta 2); plain wave-cooperative traversal against the learned-bound form at each
+    margin (outputs identical), and one run of the per-lane form unless --no-plain."""
+    import numpy as np
+    import torch
+
+    import meshlib_amd as M
+    from meshlib_amd import synth
+
+    torch.cuda.set_device(0)
+    t0 = time.time()
+    v, f = synth.bumpy_icosphere(9, 64, 5)
+    dims, xf, vox = synth.grid_for_mesh(v, (512, 512, 512))
+    nvox = 512 ** 3
+    log(f"inputs {time.time() - t0:.1f}s: {f.shape[0]} tris, 512^3 voxels")
+    t0 = time.time()
+    mesh = M.Mesh(v, f)
+    log(f"tree build + upload {time.time() - t0:.1f}s (depth {mesh.depth()})")
+    fwn = M.FastWindingNumber(mesh)
+    proj = M.PointsToMeshProjector()
+    proj.updateMeshData(mesh)
+    opts = M.DistanceToMeshOptions()
+    d_sdf = torch.empty(nvox, dtype=torch.float32, device="cuda")
+    stream = torch.cuda.current_stream()
+    sh = stream.cuda_stream
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+
+    def timed(fn):
+        ev[0].record(stream)
+        r = fn()
+        ev[1].record(stream)
+        torch.cuda.synchronize()
+        return ev[0].elapsed_time(ev[1]), r
+
+    M.set_traversal("wave_coop")
+    for _ in range(args.warmup):
+        timed(lambda: fwn.sdf_grid_device(dims, xf, opts, d_sdf.data_ptr(), stream=sh))
+    times = []
+    for i in range(args.steps):
+        t, _ = timed(lambda: fwn.sdf_grid_device(dims, xf, opts, d_sdf.data_ptr(), stream=sh))
+        times.append(t)
+        log(f"step {i}: sdf {t:.1f} ms")
+    sdf_ms = float(np.mean(times))
+    ref = d_sdf.view(torch.int32).clone()
+    plain = None
+    if not args.no_plain:
+        M.set_traversal("per_lane")
+        t, _ = timed(lambda: fwn.sdf_grid_device(dims, xf, opts, d_sdf.data_ptr(), stream=sh))
+        M.set_traversal("wave_coop")
+        plain = {"sdf_ms": t, "speedup": t / sdf_ms, "identical": bool(torch.equal(d_sdf.view(torch.int32), ref))}
+        log(f"per-lane: {t:.1f} ms")
+
+    lo, hi = v.min(0).astype(np.float64), v.max(0).astype(np.float64)
+    net = M.DistanceMLP(0.5 * (lo + hi), float(0.5 * np.max(hi - lo) * 1.2), seed=TRAIN_SEED)
+    tp = torch.from_numpy(synth.projection_queries(v, f, TRAIN_N, TRAIN_SEED)).cuda()
+    trec = torch.empty(TRAIN_N * 8, dtype=torch.int32, device="cuda")
+    proj.find_projections_device(tp.data_ptr(), TRAIN_N, trec.data_ptr(), stream=sh)
+    tlab = trec.view(torch.float32).view(-1, 8)[:, 7].sqrt().contiguous()
+    train_ms, _ = timed(lambda: net.train(tp, tlab, steps=STEPS_TRAIN[0], batch=TRAIN_BATCH, lr=TRAIN_LR,
+                                          stream=sh))
+    learned = {}
+    for m in MARGINS:
+        t, r = timed(lambda: fwn.sdf_grid_learned_device(net, m, dims, xf, opts, d_sdf.data_ptr(), stream=sh))
+        same = bool(torch.equal(d_sdf.view(torch.int32), ref))
+        learned[str(m)] = {"sdf_ms": t, "fallback_rate": r / nvox, "identical": same,
+                           "speedup_vs_plain_coop": sdf_ms / t}
+        log(f"learned m={m}: {t:.1f} ms, fallback {r / nvox:.4f}, identical {same}")
+    out = {"metric": "signed-dist voxels/s (offset-scale volume)", "value": nvox / (sdf_ms / 1e3),
+           "unit": "voxels/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": sdf_ms,
+           "higher_is_better": True, "data": "synthetic", "dtype": "f32+f64",
+           "config": {"workload": "bumpy icosphere k=9 (5,242,880 tris, 64 bumps): 512^3 signed-distance grid",
+                      "voxels": nvox, "beta": 2.0},
+           "vs_per_lane": plain,
+           "learned_vs_plain": {"train_ms": train_ms, "train_steps": STEPS_TRAIN[0], "margins": learned}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -198,13 +301,18 @@ def main():
                     help="skip the one-step comparison run with the per-lane (reference-form) traversal")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-bound leg")
     ap.add_argument("--train-steps", type=int, default=2000)
-    ap.add_argument("--workload", default="single", choices=["single", "batched"],
-                    help="single: the k=8 mesh step (default); batched: the uneven multi-mesh batch")
+    ap.add_argument("--workload", default="single", choices=["single", "batched", "offset"],
+                    help="single: the k=8 mesh step (default); batched: the uneven multi-mesh batch; "
+                         "offset: the k=9 512^3 volume")
     ap.add_argument("--items", type=int, default=64, help="meshes in the batched workload")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run one 1 GiB device-to-device copy first (reference kernel for PMC byte counters)")
     args = ap.parse_args()
     STEPS_TRAIN[0] = args.train_steps
     if args.workload == "batched" and args.gpus == 1:
         return batched_main(args)
+    if args.workload == "offset" and args.gpus == 1:
+        return offset_main(args)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: this build has no multi-GPU path (asked for --gpus "
               f"{args.gpus}); the path shards by z-slabs / query ranges with no exchange, see DESIGN.md",
@@ -218,6 +326,8 @@ def main():
     from meshlib_amd import synth
 
     torch.cuda.set_device(0)
+    if args.copy_calibration:
+        calibration_copy(torch)
     t0 = time.time()
     v, f = synth.bumpy_icosphere(MESH_LEVEL, MESH_BUMPS, MESH_SEED)
     dims, xf, vox = synth.grid_for_mesh(v, (GRID, GRID, GRID))
@@ -321,6 +431,15 @@ def main():
             n_p += sel.size
         cpu_vox_rate = n_v / t_v
         cpu_pts_rate = n_p / t_p
+        # the same sample kind on one thread, for the per-core figure
+        ids1 = rng.choice(nvox, 4000, replace=False).astype(np.int64)
+        t = time.perf_counter()
+        om.sdf_grid_sample(dims, xf, ids1, nthreads=1)
+        vox1 = ids1.size / (time.perf_counter() - t)
+        sel1 = rng.choice(NQ, 8000, replace=False)
+        t = time.perf_counter()
+        om.find_projections(pts[sel1], nthreads=1)
+        pts1 = sel1.size / (time.perf_counter() - t)
         cpu_step = nvox / cpu_vox_rate + NQ / cpu_pts_rate
         cv_d, cp_d = cv.as_dict(), cp.as_dict()
         # algorithmic bytes per unit (reference AoS record sizes): query 12 + result 32, 32 B per
@@ -332,16 +451,22 @@ def main():
         cpu = {"value": (nvox + NQ) / cpu_step, "unit": "queries/s", "cores": cores, "kind": "port",
                "sample": f"{n_v} seeded voxels of the 256^3 grid + {n_p} seeded queries of the 2^22 set, "
                          f"{t_v + t_p:.1f}s of CPU time; step time extrapolated as voxels/rate + points/rate",
-               "voxels_per_s": cpu_vox_rate, "pts_per_s": cpu_pts_rate}
+               "voxels_per_s": cpu_vox_rate, "pts_per_s": cpu_pts_rate,
+               "one_thread": {"voxels_per_s": vox1, "pts_per_s": pts1,
+                              "sample": f"{ids1.size} voxels + {sel1.size} queries"}}
         log(f"cpu baseline: {cpu_vox_rate:.0f} vox/s, {cpu_pts_rate:.0f} pts/s on {cores} threads")
 
     roof = None
     if b_vox is not None:
         achieved = b_vox * nvox / (sdf_avg / 1000.0) / 1e9
         kname = "kSdfGridCoop" if args.traversal == "wave_coop" else "kSdfGrid"
+        traffic = pmc_traffic_bytes() if args.traversal == "wave_coop" else None
         roof = {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                "bytes_per_voxel": b_vox, "bytes_per_point": b_pt}
+                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                "bytes_per_voxel": b_vox, "bytes_per_point": b_pt,
+                "algorithmic_bytes_per_launch": b_vox * nvox,
+                "hbm_gbs_measured": (traffic / (sdf_avg / 1000.0) / 1e9) if traffic else None,
+                "l2_hit_rate": PMC_SDF_L2_HIT if traffic else None}
     out = {
         "metric": "signed-dist voxels/s & projected pts/s; learned-bound vs plain GPU speedup",
         "value": value, "unit": "queries/s (voxels + projected points)",

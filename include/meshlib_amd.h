@@ -94,17 +94,28 @@ int mi_set_memory_limit(size_t bytes);
 int mi_set_traversal(int mode);
 int mi_get_traversal(void);
 
+/* Where mi_mesh_create builds the AABB tree and dipoles, process-wide.  The reference builds them
+ * on the host inside the first call (source/MRCuda/MRCudaFastWindingNumber.cpp:70, AABBTree ctor
+ * source/MRMesh/MRAABBTree.cpp:31-60, calcDipoles source/MRMesh/MRDipole.cpp:23-71); both builders
+ * here produce that tree node-for-node and bit-for-bit.
+ *   MI_TREE_BUILDER_DEVICE level-synchronous GPU build (default)
+ *   MI_TREE_BUILDER_HOST   multithreaded host build, flattened and uploaded */
+#define MI_TREE_BUILDER_DEVICE 0
+#define MI_TREE_BUILDER_HOST 1
+int mi_set_tree_builder(int builder);
+int mi_get_tree_builder(void);
+
 /* ---------------------------------------------------------------- mesh data */
 
 /* Cuda::FastWindingNumber::prepareData_ (source/MRCuda/MRCudaFastWindingNumber.cpp:46-85)
  * and Cuda::PointsToMeshProjector::updateMeshData (source/MRCuda/MRCudaPointsToMeshProjector.cpp:31-50):
  * verts float[nv*3], tris int32[nf*3] (getTriangulation order).  Builds the AABB tree
- * (source/MRMesh/MRAABBTree.cpp:31-60) and dipoles (source/MRMesh/MRDipole.cpp:23-71) on the
- * host and uploads the flattened tree to the current HIP device. */
+ * (source/MRMesh/MRAABBTree.cpp:31-60) and dipoles (source/MRMesh/MRDipole.cpp:23-71) with the
+ * selected builder and keeps the flattened tree on the current HIP device.  At most 2^30 faces. */
 int  mi_mesh_create(const float* verts, int32_t nv, const int32_t* tris, int32_t nf, mi_mesh** out);
 void mi_mesh_destroy(mi_mesh* m);
 int  mi_mesh_num_nodes(const mi_mesh* m);
-/* exports the host tree / dipoles in the reference layout (AABBTree::nodes(), Mesh::getDipoles()) */
+/* exports the tree / dipoles in the reference layout (AABBTree::nodes(), Mesh::getDipoles()) */
 int  mi_mesh_get_tree(const mi_mesh* m, mi_tree_node* nodes, mi_dipole* dipoles);
 /* depth of the tree (longest root-to-leaf path, in edges) */
 int  mi_mesh_depth(const mi_mesh* m);
@@ -218,6 +229,15 @@ int  mi_batch_autotune(mi_batch* b, const mi_batch_item* items, int32_t count, c
                        float upDistLimitSq, void* stream, float* bestMs);
 int  mi_batch_set_config(mi_batch* b, int32_t blocksPerCU, int32_t xcdSplit);
 int  mi_batch_get_config(mi_batch* b, int32_t* blocksPerCU, int32_t* xcdSplit);
+/* the batch with per-query caps (squared) on the search radius: d_capSdf[i] float[voxels of item i]
+ * and d_capProj[i] float[numPts of item i] (device; either array or entry may be NULL = no cap).
+ * Pass 1 is one persistent launch with the caps; pass 2 one persistent launch, uncapped, over the
+ * queries that found nothing below a cap smaller than the plain limit (*reruns = their count).
+ * Any cap at or above a query's true distance gives the plain result, so outputs equal
+ * mi_batch_run_device's.  Synchronises the stream between the passes. */
+int  mi_batch_run_capped_device(mi_batch* b, const mi_batch_item* items, int32_t count, const float* const* d_capSdf,
+                                const float* const* d_capProj, const mi_dist_options* opts, float upDistLimitSq,
+                                void* stream, int64_t* reruns);
 
 /* ---------------------------------------------------------------- learned distance bound
  * No counterpart in the reference: a Fourier-feature MLP (99 features -> 4 x 128 SiLU -> softplus)
@@ -247,6 +267,11 @@ int  mi_proj_find_projections_learned_device(mi_mesh* mesh, mi_mlp* m, float mar
 int  mi_sdf_grid_learned_device(mi_mesh* mesh, mi_mlp* m, float margin, const int32_t dims[3], const float xf[12],
                                 const mi_dist_options* opts, int32_t zBegin, int32_t zEnd, float* d_res,
                                 int64_t* reruns, void* stream);
+/* the batch (mi_batch_run_capped_device) with caps (pred * (1 + margin))^2 from each item's own
+ * network nets[i] (NULL = no cap for that item); exact */
+int  mi_batch_run_learned_device(mi_batch* b, const mi_batch_item* items, int32_t count, mi_mlp* const* nets,
+                                 float margin, const mi_dist_options* opts, float upDistLimitSq, void* stream,
+                                 int64_t* reruns);
 
 #ifdef __cplusplus
 }

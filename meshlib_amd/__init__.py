@@ -94,6 +94,8 @@ def lib():
         "mi_set_memory_limit": (C.c_int, [SZ]),
         "mi_set_traversal": (C.c_int, [C.c_int]),
         "mi_get_traversal": (C.c_int, []),
+        "mi_set_tree_builder": (C.c_int, [C.c_int]),
+        "mi_get_tree_builder": (C.c_int, []),
         "mi_mesh_create": (C.c_int, [P, I32, P, I32, C.POINTER(P)]),
         "mi_mesh_destroy": (None, [P]),
         "mi_mesh_num_nodes": (C.c_int, [P]),
@@ -121,6 +123,8 @@ def lib():
         "mi_batch_autotune": (C.c_int, [P, P, I32, P, F, P, C.POINTER(F)]),
         "mi_batch_set_config": (C.c_int, [P, I32, I32]),
         "mi_batch_get_config": (C.c_int, [P, C.POINTER(I32), C.POINTER(I32)]),
+        "mi_batch_run_capped_device": (C.c_int, [P, P, I32, P, P, P, F, P, C.POINTER(I64)]),
+        "mi_batch_run_learned_device": (C.c_int, [P, P, I32, P, F, P, F, P, C.POINTER(I64)]),
         "mi_mlp_create": (C.c_int, [P, F, C.c_uint64, C.POINTER(P)]),
         "mi_mlp_destroy": (None, [P]),
         "mi_mlp_num_params": (C.c_int, []),
@@ -193,6 +197,20 @@ def set_traversal(mode: str):
 def get_traversal() -> str:
     v = int(lib().mi_get_traversal())
     return {b: a for a, b in TRAVERSALS.items()}[v]
+
+
+TREE_BUILDERS = {"device": 0, "host": 1}
+
+
+def set_tree_builder(builder: str):
+    """Where Mesh() builds the AABB tree and dipoles: 'device' (GPU, default) or 'host'.  Both give
+    the reference's tree node-for-node and bit-for-bit."""
+    _check(lib().mi_set_tree_builder(TREE_BUILDERS[builder]))
+
+
+def get_tree_builder() -> str:
+    v = int(lib().mi_get_tree_builder())
+    return {b: a for a, b in TREE_BUILDERS.items()}[v]
 
 
 def set_memory_limit(nbytes: int):
@@ -324,6 +342,42 @@ class Batch:
         _check(lib().mi_batch_autotune(self._h, arr, len(items), C.byref(o), upDistLimitSq,
                                        C.c_void_p(stream or None), C.byref(ms)))
         return float(ms.value)
+
+    def run_capped(self, items, cap_sdf=None, cap_proj=None, options: DistanceToMeshOptions = None,
+                   upDistLimitSq: float = FLT_MAX, stream: int = 0) -> int:
+        """Two persistent launches with per-query caps (device pointers per item, 0/None = no cap);
+        returns the number of rerun queries.  Outputs equal run()'s."""
+        arr = self._items(items)
+        n = len(items)
+
+        def ptrs(lst):
+            if lst is None:
+                return None
+            a = (C.c_void_p * n)()
+            for i, x in enumerate(lst):
+                a[i] = x or None
+            return a
+        cs, cp = ptrs(cap_sdf), ptrs(cap_proj)
+        o = (options or DistanceToMeshOptions())._c()
+        r = C.c_int64(0)
+        _check(lib().mi_batch_run_capped_device(self._h, arr, n, cs, cp, C.byref(o), upDistLimitSq,
+                                                C.c_void_p(stream or None), C.byref(r)))
+        return int(r.value)
+
+    def run_learned(self, items, nets, margin: float, options: DistanceToMeshOptions = None,
+                    upDistLimitSq: float = FLT_MAX, stream: int = 0) -> int:
+        """The batch with caps (pred * (1 + margin))^2 from each item's DistanceMLP (None = no cap);
+        returns the number of rerun queries.  Outputs equal run()'s."""
+        arr = self._items(items)
+        n = len(items)
+        hs = (C.c_void_p * n)()
+        for i, net in enumerate(nets):
+            hs[i] = net.handle.value if net is not None else None
+        o = (options or DistanceToMeshOptions())._c()
+        r = C.c_int64(0)
+        _check(lib().mi_batch_run_learned_device(self._h, arr, n, hs, margin, C.byref(o), upDistLimitSq,
+                                                 C.c_void_p(stream or None), C.byref(r)))
+        return int(r.value)
 
     def config(self):
         a, b = C.c_int32(0), C.c_int32(0)

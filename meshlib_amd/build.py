@@ -21,7 +21,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmeshlib_amd.so")
 ARCH = os.environ.get("MESHLIB_AMD_ARCH", "gfx950")
 
-SOURCES = ["mi_tree.cpp", "mi_kernels.hip", "mi_capi.cpp", "mi_mlp.hip", "mi_mlp_capi.cpp"]
+SOURCES = ["mi_tree.cpp", "mi_tree_gpu.hip", "mi_kernels.hip", "mi_capi.cpp", "mi_mlp.hip", "mi_mlp_capi.cpp"]
 
 
 def hipcc() -> str:

@@ -5,6 +5,7 @@
 
 #include "mi_kernels.hpp"
 #include "mi_tree.hpp"
+#include "mi_tree_gpu.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -22,6 +23,7 @@ namespace {
 
 thread_local std::string g_err;
 std::atomic<size_t> g_memLimit{ 0 };
+std::atomic<int> g_treeBuilder{ MI_TREE_BUILDER_DEVICE };
 
 int fail(int code, const std::string& msg)
 {
@@ -154,7 +156,9 @@ int mi_internal_fail(int code, const char* msg) { return fail(code, msg); }
 struct mi_mesh {
     int device = 0;
     int32_t nv = 0, nf = 0;
-    mi::HostTree tree;
+    int32_t numNodes = 0, depth = 0;
+    mi::HostTree tree;     // host build: filled at creation; device build: downloaded on request
+    DevBuf treeDev;        // device build: nodes + dipoles in the reference layout
     int stackD = 8;
     DevBuf arena;
     mi::MeshDev dev{};
@@ -215,60 +219,146 @@ int mi_set_traversal(int mode)
 
 int mi_get_traversal(void) { return static_cast<int>(mi::traversal()); }
 
+int mi_set_tree_builder(int builder)
+{
+    if (builder != MI_TREE_BUILDER_DEVICE && builder != MI_TREE_BUILDER_HOST)
+        return fail(MI_E_INVALID, "unknown tree builder");
+    g_treeBuilder.store(builder);
+    return MI_OK;
+}
+
+int mi_get_tree_builder(void) { return g_treeBuilder.load(); }
+
+} // extern "C"
+
+namespace {
+
+struct ArenaLayout {
+    size_t dnode, tri, dint, dleaf;
+    size_t total() const { return dnode + tri + dint + dleaf; }
+};
+
+ArenaLayout arenaLayout(int32_t nf)
+{
+    const size_t ni = nf > 0 ? (size_t)nf - 1 : 0, nl = nf > 0 ? (size_t)nf : 0;
+    return { align256(ni * 64), align256(nl * 48), align256(ni * 32), align256(nl * 32) };
+}
+
+void bindArena(mi_mesh* m, const ArenaLayout& L)
+{
+    char* base = m->arena.as<char>();
+    m->dev.dnode = reinterpret_cast<const float4*>(base);
+    m->dev.tri = reinterpret_cast<const float4*>(base + L.dnode);
+    m->dev.dint = reinterpret_cast<const float4*>(base + L.dnode + L.tri);
+    m->dev.dleaf = reinterpret_cast<const float4*>(base + L.dnode + L.tri + L.dint);
+}
+
+void setRoot(mi_mesh* m, const mi::Box3& rootBox, int32_t rootRef)
+{
+    for (int i = 0; i < 3; ++i) {
+        m->dev.rootMin[i] = rootBox.min[i];
+        m->dev.rootMax[i] = rootBox.max[i];
+        const float c = 0.5f * (rootBox.min[i] + rootBox.max[i]);
+        const float h = 0.75f * (rootBox.max[i] - rootBox.min[i]);
+        m->sortLo[i] = c - h;
+        m->sortHi[i] = c + h;
+    }
+    m->dev.rootRef = rootRef;
+}
+
+// tree built by the host builder, flattened on the host and uploaded
+int createHostBuilt(mi_mesh* m, const float* verts, const int32_t* tris)
+{
+    for (int64_t i = 0; i < 3 * (int64_t)m->nf; ++i)
+        if (tris[i] < 0 || tris[i] >= m->nv) return fail(MI_E_INVALID, "triangle vertex index out of range");
+    try {
+        mi::buildHostTree(verts, m->nv, tris, m->nf, m->tree);
+    } catch (const std::bad_alloc&) {
+        return fail(MI_E_NOMEM, "out of host memory building the tree");
+    }
+    m->depth = m->tree.depth;
+    m->stackD = mi::stackDepthFor(m->depth);
+    if (m->stackD == 0) return fail(MI_E_LIMIT, "tree deeper than 32 levels");
+    mi::FlatTree ft;
+    mi::flattenTree(m->tree, verts, tris, ft);
+    const ArenaLayout L = arenaLayout(m->nf);
+    if (L.total()) {
+        MI_HIP(m->arena.alloc(L.total()));
+        bindArena(m, L);
+        char* base = m->arena.as<char>();
+        if (!ft.dnode.empty()) MI_HIP(hipMemcpy(base, ft.dnode.data(), ft.dnode.size() * 4, hipMemcpyHostToDevice));
+        MI_HIP(hipMemcpy(base + L.dnode, ft.tri.data(), ft.tri.size() * 4, hipMemcpyHostToDevice));
+        if (!ft.dint.empty())
+            MI_HIP(hipMemcpy(base + L.dnode + L.tri, ft.dint.data(), ft.dint.size() * 4, hipMemcpyHostToDevice));
+        MI_HIP(hipMemcpy(base + L.dnode + L.tri + L.dint, ft.dleaf.data(), ft.dleaf.size() * 4,
+                         hipMemcpyHostToDevice));
+    }
+    if (m->nf > 0) setRoot(m, ft.rootBox, ft.rootRef);
+    return MI_OK;
+}
+
+// tree, dipoles and the flattened layout built by the device builder from uploaded mesh arrays
+int createDeviceBuilt(mi_mesh* m, const float* verts, const int32_t* tris)
+{
+    if (m->nf == 0) return MI_OK;
+    const size_t nn = 2 * (size_t)m->nf - 1;
+    const ArenaLayout L = arenaLayout(m->nf);
+    MI_HIP(m->arena.alloc(L.total()));
+    bindArena(m, L);
+    MI_HIP(m->treeDev.alloc(2 * align256(nn * sizeof(mi::TreeNode))));
+    DevBuf dv, dt, scratch;
+    MI_HIP(dv.alloc(std::max<size_t>((size_t)m->nv * 12, 16)));
+    MI_HIP(dt.alloc((size_t)m->nf * 12));
+    MI_HIP(scratch.alloc(mi::deviceTreeScratchBytes(m->nf)));
+    if (m->nv > 0) MI_HIP(hipMemcpyAsync(dv.p, verts, (size_t)m->nv * 12, hipMemcpyHostToDevice, m->stream));
+    MI_HIP(hipMemcpyAsync(dt.p, tris, (size_t)m->nf * 12, hipMemcpyHostToDevice, m->stream));
+    char* base = m->arena.as<char>();
+    mi::DeviceTreeOut o;
+    o.nodes = m->treeDev.as<mi::TreeNode>();
+    o.dipoles = reinterpret_cast<mi::Dipole*>(m->treeDev.as<char>() + align256(nn * sizeof(mi::TreeNode)));
+    o.dnode = reinterpret_cast<float4*>(base);
+    o.tri = reinterpret_cast<float4*>(base + L.dnode);
+    o.dint = reinterpret_cast<float4*>(base + L.dnode + L.tri);
+    o.dleaf = reinterpret_cast<float4*>(base + L.dnode + L.tri + L.dint);
+    mi::DeviceTreeInfo info;
+    const hipError_t e = mi::buildDeviceTree(dv.as<float>(), m->nv, dt.as<int32_t>(), m->nf, scratch.p, scratch.bytes,
+                                             o, info, m->stream);
+    if (e == hipErrorInvalidValue) return fail(MI_E_INVALID, "triangle vertex index out of range");
+    MI_HIP(e);
+    m->depth = info.depth;
+    m->stackD = mi::stackDepthFor(m->depth);
+    if (m->stackD == 0) return fail(MI_E_LIMIT, "tree deeper than 32 levels");
+    setRoot(m, info.rootBox, info.rootRef);
+    return MI_OK;
+}
+
+} // namespace
+
+extern "C" {
+
 int mi_mesh_create(const float* verts, int32_t nv, const int32_t* tris, int32_t nf, mi_mesh** out)
 {
     if (!out) return fail(MI_E_INVALID, "out is null");
     *out = nullptr;
     if (nv < 0 || nf < 0 || (nv > 0 && !verts) || (nf > 0 && !tris))
         return fail(MI_E_INVALID, "invalid mesh arrays");
-    for (int64_t i = 0; i < 3 * (int64_t)nf; ++i)
-        if (tris[i] < 0 || tris[i] >= nv) return fail(MI_E_INVALID, "triangle vertex index out of range");
+    if (nf > (1 << 30)) return fail(MI_E_LIMIT, "more than 2^30 faces");
     mi_mesh* m = new (std::nothrow) mi_mesh();
     if (!m) return fail(MI_E_NOMEM, "out of host memory");
     m->nv = nv;
     m->nf = nf;
-    auto cleanup = [&](int code) { delete m; return code; };
-    if (hipGetDevice(&m->device) != hipSuccess) return cleanup(fail(MI_E_HIP, "AMD GPU error: no device"));
-    if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess)
-        return cleanup(fail(MI_E_HIP, "AMD GPU error: stream creation failed"));
-    try {
-        mi::buildHostTree(verts, nv, tris, nf, m->tree);
-    } catch (const std::bad_alloc&) {
-        return cleanup(fail(MI_E_NOMEM, "out of host memory building the tree"));
+    m->numNodes = nf > 0 ? 2 * nf - 1 : 0;
+    if (hipGetDevice(&m->device) != hipSuccess) { delete m; return fail(MI_E_HIP, "AMD GPU error: no device"); }
+    if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete m;
+        return fail(MI_E_HIP, "AMD GPU error: stream creation failed");
     }
-    m->stackD = mi::stackDepthFor(m->tree.depth);
-    if (m->stackD == 0) return cleanup(fail(MI_E_LIMIT, "tree deeper than 32 levels"));
-    mi::FlatTree ft;
-    mi::flattenTree(m->tree, verts, tris, ft);
-    const size_t bDnode = align256(ft.dnode.size() * 4), bTri = align256(ft.tri.size() * 4);
-    const size_t bDint = align256(ft.dint.size() * 4), bDleaf = align256(ft.dleaf.size() * 4);
-    const size_t total = bDnode + bTri + bDint + bDleaf;
-    if (total) {
-        if (m->arena.alloc(total) != hipSuccess)
-            return cleanup(fail(MI_E_HIP, "AMD GPU error: cannot allocate the tree arena"));
-        char* base = m->arena.as<char>();
-        hipError_t e = hipSuccess;
-        if (!ft.dnode.empty()) e = hipMemcpy(base, ft.dnode.data(), ft.dnode.size() * 4, hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemcpy(base + bDnode, ft.tri.data(), ft.tri.size() * 4, hipMemcpyHostToDevice);
-        if (e == hipSuccess && !ft.dint.empty())
-            e = hipMemcpy(base + bDnode + bTri, ft.dint.data(), ft.dint.size() * 4, hipMemcpyHostToDevice);
-        if (e == hipSuccess)
-            e = hipMemcpy(base + bDnode + bTri + bDint, ft.dleaf.data(), ft.dleaf.size() * 4, hipMemcpyHostToDevice);
-        if (e != hipSuccess) return cleanup(fail(MI_E_HIP, std::string("AMD GPU error: ") + hipGetErrorString(e)));
-        m->dev.dnode = reinterpret_cast<const float4*>(base);
-        m->dev.tri = reinterpret_cast<const float4*>(base + bDnode);
-        m->dev.dint = reinterpret_cast<const float4*>(base + bDnode + bTri);
-        m->dev.dleaf = reinterpret_cast<const float4*>(base + bDnode + bTri + bDint);
+    const int rc = g_treeBuilder.load() == MI_TREE_BUILDER_HOST ? createHostBuilt(m, verts, tris)
+                                                                 : createDeviceBuilt(m, verts, tris);
+    if (rc != MI_OK) {
+        delete m;
+        return rc;
     }
-    for (int i = 0; i < 3; ++i) {
-        m->dev.rootMin[i] = ft.rootBox.min[i];
-        m->dev.rootMax[i] = ft.rootBox.max[i];
-        const float c = 0.5f * (ft.rootBox.min[i] + ft.rootBox.max[i]);
-        const float h = 0.75f * (ft.rootBox.max[i] - ft.rootBox.min[i]);
-        m->sortLo[i] = c - h;
-        m->sortHi[i] = c + h;
-    }
-    m->dev.rootRef = ft.rootRef;
     m->dev.numFaces = nf;
     *out = m;
     return MI_OK;
@@ -281,9 +371,9 @@ void mi_mesh_destroy(mi_mesh* m)
     delete m;
 }
 
-int mi_mesh_num_nodes(const mi_mesh* m) { return m ? (int)m->tree.nodes.size() : 0; }
+int mi_mesh_num_nodes(const mi_mesh* m) { return m ? m->numNodes : 0; }
 
-int mi_mesh_depth(const mi_mesh* m) { return m ? m->tree.depth : 0; }
+int mi_mesh_depth(const mi_mesh* m) { return m ? m->depth : 0; }
 
 int mi_build_tree_host(const float* verts, int32_t nv, const int32_t* tris, int32_t nf, mi_tree_node* nodes,
                        mi_dipole* dipoles, int32_t* depth)
@@ -310,6 +400,13 @@ int mi_mesh_get_tree(const mi_mesh* m, mi_tree_node* nodes, mi_dipole* dipoles)
     if (!m) return fail(MI_E_INVALID, "mesh is null");
     static_assert(sizeof(mi_tree_node) == sizeof(mi::TreeNode), "node layout");
     static_assert(sizeof(mi_dipole) == sizeof(mi::Dipole), "dipole layout");
+    if (m->treeDev.p) {
+        const size_t nb = (size_t)m->numNodes * sizeof(mi_tree_node);
+        MI_HIP(hipSetDevice(m->device));
+        if (nodes) MI_HIP(hipMemcpy(nodes, m->treeDev.p, nb, hipMemcpyDeviceToHost));
+        if (dipoles) MI_HIP(hipMemcpy(dipoles, m->treeDev.as<char>() + align256(nb), nb, hipMemcpyDeviceToHost));
+        return MI_OK;
+    }
     if (nodes && !m->tree.nodes.empty())
         std::memcpy(nodes, m->tree.nodes.data(), m->tree.nodes.size() * sizeof(mi_tree_node));
     if (dipoles && !m->tree.dipoles.empty())
@@ -742,14 +839,25 @@ struct mi_batch {
     DevBuf items;        // BatchItem[]
     DevBuf tickets;      // 8 x u64
     DevBuf sortWs;       // per-item Morton sort workspaces
+    DevBuf rerunWs;      // capped form: per-item rerun counters and lists
     std::vector<mi::BatchItem> host;
     std::mutex mu;
 };
 
 namespace {
 
+int uploadItems(mi_batch* b, int32_t count, hipStream_t s)
+{
+    if ((size_t)count * sizeof(mi::BatchItem) > b->items.bytes)
+        MI_HIP(b->items.alloc((size_t)count * sizeof(mi::BatchItem)));
+    // pageable source: the copy is complete when hipMemcpyAsync returns
+    MI_HIP(hipMemcpyAsync(b->items.p, b->host.data(), (size_t)count * sizeof(mi::BatchItem), hipMemcpyHostToDevice, s));
+    return MI_OK;
+}
+
 // builds the device descriptors (sorting every item's points) and returns the unit count
-int prepareBatch(mi_batch* b, const mi_batch_item* items, int32_t count, hipStream_t s, int64_t* total, int* stackD)
+int prepareBatch(mi_batch* b, const mi_batch_item* items, int32_t count, hipStream_t s, int64_t* total, int* stackD,
+                 bool upload = true)
 {
     *total = 0;
     *stackD = 8;
@@ -795,12 +903,8 @@ int prepareBatch(mi_batch* b, const mi_batch_item* items, int32_t count, hipStre
         d.unitBegin = unit;
         unit += d.gridUnits + d.projUnits;
     }
-    if ((size_t)count * sizeof(mi::BatchItem) > b->items.bytes)
-        MI_HIP(b->items.alloc((size_t)count * sizeof(mi::BatchItem)));
-    // pageable source: the copy is complete when hipMemcpyAsync returns
-    MI_HIP(hipMemcpyAsync(b->items.p, b->host.data(), (size_t)count * sizeof(mi::BatchItem), hipMemcpyHostToDevice, s));
     *total = unit;
-    return MI_OK;
+    return upload ? uploadItems(b, count, s) : MI_OK;
 }
 
 int launchPrepared(mi_batch* b, int32_t count, int64_t total, int stackD, const mi_dist_options* opts, float up,
@@ -871,6 +975,71 @@ int mi_batch_run_device(mi_batch* b, const mi_batch_item* items, int32_t count, 
     int stackD = 8;
     if (int rc = prepareBatch(b, items, count, s, &total, &stackD)) return rc;
     return launchPrepared(b, count, total, stackD, opts, upDistLimitSq, b->blocksPerCU, b->xcdSplit, s);
+}
+
+int mi_batch_run_capped_device(mi_batch* b, const mi_batch_item* items, int32_t count, const float* const* d_capSdf,
+                               const float* const* d_capProj, const mi_dist_options* opts, float upDistLimitSq,
+                               void* stream, int64_t* reruns)
+{
+    if (int rc = checkBatch(b, items, count, opts)) return rc;
+    if (reruns) *reruns = 0;
+    if (count == 0) return MI_OK;
+    std::lock_guard<std::mutex> lk(b->mu);
+    MI_HIP(hipSetDevice(b->device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    int64_t total = 0;
+    int stackD = 8;
+    if (int rc = prepareBatch(b, items, count, s, &total, &stackD, false)) return rc;
+    // rerun storage: 2 counters per item, then per item a voxel list and a point list
+    const size_t cntBytes = align256((size_t)count * 8);
+    size_t need = cntBytes;
+    for (int32_t i = 0; i < count; ++i) {
+        const mi_batch_item& it = items[i];
+        need += align256((size_t)it.dims[0] * it.dims[1] * it.dims[2] * 4) + align256((size_t)it.numPts * 4);
+    }
+    if (b->rerunWs.bytes < need) MI_HIP(b->rerunWs.alloc(need));
+    char* base = b->rerunWs.as<char>();
+    uint32_t* counts = reinterpret_cast<uint32_t*>(base);
+    size_t off = cntBytes;
+    for (int32_t i = 0; i < count; ++i) {
+        const mi_batch_item& it = items[i];
+        mi::BatchItem& d = b->host[i];
+        const size_t nvox = (size_t)it.dims[0] * it.dims[1] * it.dims[2];
+        d.capSdf = d_capSdf ? d_capSdf[i] : nullptr;
+        d.capProj = d_capProj ? d_capProj[i] : nullptr;
+        d.rerunVox = reinterpret_cast<uint32_t*>(base + off);
+        off += align256(nvox * 4);
+        d.rerunPts = reinterpret_cast<uint32_t*>(base + off);
+        off += align256((size_t)it.numPts * 4);
+        d.rerunCount = counts + 2 * i;
+    }
+    MI_HIP(mi::fillU32(counts, 2 * count, 0u, s));
+    if (int rc = uploadItems(b, count, s)) return rc;
+    if (int rc = launchPrepared(b, count, total, stackD, opts, upDistLimitSq, b->blocksPerCU, b->xcdSplit, s)) return rc;
+    // second pass: one persistent launch over the queries whose cap was too small, uncapped
+    std::vector<uint32_t> hc((size_t)count * 2);
+    MI_HIP(hipMemcpyAsync(hc.data(), counts, hc.size() * 4, hipMemcpyDeviceToHost, s));
+    MI_HIP(hipStreamSynchronize(s));
+    int64_t unit = 0, again = 0;
+    for (int32_t i = 0; i < count; ++i) {
+        mi::BatchItem& d = b->host[i];
+        const uint32_t nv = hc[2 * i], np = hc[2 * i + 1];
+        again += (int64_t)nv + np;
+        d.voxList = d.rerunVox;
+        d.nVoxList = nv;
+        d.gridUnits = (nv + 63) / 64;
+        d.perm = d.rerunPts;
+        d.npts = np;
+        d.projUnits = (np + 63) / 64;
+        d.capSdf = d.capProj = nullptr;
+        d.rerunVox = d.rerunPts = d.rerunCount = nullptr;
+        d.unitBegin = unit;
+        unit += d.gridUnits + d.projUnits;
+    }
+    if (reruns) *reruns = again;
+    if (unit == 0) return MI_OK;
+    if (int rc = uploadItems(b, count, s)) return rc;
+    return launchPrepared(b, count, unit, stackD, opts, upDistLimitSq, b->blocksPerCU, b->xcdSplit, s);
 }
 
 int mi_batch_autotune(mi_batch* b, const mi_batch_item* items, int32_t count, const mi_dist_options* opts,

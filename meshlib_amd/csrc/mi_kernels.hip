@@ -862,22 +862,45 @@ __global__ __launch_bounds__(kBlock) void kBatch(const BatchItem* __restrict__ i
         const BatchItem& it = items[uniform(lo)];
         const int64_t lu = u - it.unitBegin;
         if (lu < it.gridUnits) {
-            int x, y, z;
-            const bool valid = brickVoxel((int)lu, lane, it.nbx, it.nby, it.nbz, it.dx, it.dy, 0, it.dz, x, y, z);
+            int x = 0, y = 0, z = 0;
+            bool valid;
+            size_t idx = 0;
+            if (it.voxList) {
+                const int64_t i = lu * 64 + lane;
+                valid = i < it.nVoxList;
+                if (valid) {
+                    idx = it.voxList[i];
+                    x = (int)(idx % (size_t)it.dx);
+                    y = (int)((idx / (size_t)it.dx) % (size_t)it.dy);
+                    z = (int)(idx / ((size_t)it.dx * it.dy));
+                }
+            } else {
+                valid = brickVoxel((int)lu, lane, it.nbx, it.nby, it.nbz, it.dx, it.dy, 0, it.dz, x, y, z);
+                idx = valid ? ((size_t)z * it.dy + y) * it.dx + x : 0;
+            }
             float px = (float)x, py = (float)y, pz = (float)z;
             xfApply(it.xf, px, py, pz);
-            const float resSq = coopDistance<D>(it.m, px, py, pz, valid ? o.maxDistSq : -1.f, stkMem[w]);
-            const size_t idx = valid ? ((size_t)z * it.dy + y) * it.dx + x : 0;
-            sdfStore(it.m, px, py, pz, valid, resSq, o, reinterpret_cast<uint4*>(stkMem[w]), it.sdf, idx);
+            float lim = o.maxDistSq;
+            if (it.capSdf && valid) { const float c = it.capSdf[idx]; lim = c < lim ? c : lim; }
+            const float resSq = coopDistance<D>(it.m, px, py, pz, valid ? lim : -1.f, stkMem[w]);
+            const bool again = it.rerunVox && valid && !(resSq < lim) && lim < o.maxDistSq;
+            if (it.rerunVox) appendWave(again, (uint32_t)idx, it.rerunVox, it.rerunCount);
+            sdfStore(it.m, px, py, pz, valid && !again, resSq, o, reinterpret_cast<uint4*>(stkMem[w]), it.sdf, idx);
         } else {
             const int64_t i = (lu - it.gridUnits) * 64 + lane;
             const bool valid = i < it.npts;
             const int64_t q = valid ? (it.perm ? (int64_t)it.perm[i] : i) : 0;
             float px = 0, py = 0, pz = 0;
-            if (valid) { px = it.pts[3 * q]; py = it.pts[3 * q + 1]; pz = it.pts[3 * q + 2]; }
-            BestHit h;
-            coopProject<D>(it.m, px, py, pz, up, valid, stkMem[w], h);
+            float lim = up;
             if (valid) {
+                px = it.pts[3 * q]; py = it.pts[3 * q + 1]; pz = it.pts[3 * q + 2];
+                if (it.capProj) { const float c = it.capProj[q]; lim = c < up ? c : up; }
+            }
+            BestHit h;
+            coopProject<D>(it.m, px, py, pz, lim, valid, stkMem[w], h);
+            const bool again = it.rerunPts && valid && h.face < 0 && lim < up;
+            if (it.rerunPts) appendWave(again, (uint32_t)q, it.rerunPts, it.rerunCount + 1);
+            if (valid && !again) {
                 float4* op = reinterpret_cast<float4*>(it.proj + q);
                 op[0] = make_float4(__int_as_float(h.face), h.px, h.py, h.pz);
                 op[1] = make_float4(__int_as_float(-1), h.a, h.b, h.distSq);

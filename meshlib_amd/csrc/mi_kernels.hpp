@@ -43,6 +43,17 @@ struct BatchItem {
     int64_t npts;
     ProjRec* proj;
     int64_t gridUnits, projUnits, unitBegin;
+    // list mode: the grid part covers only these voxel ids (64 per unit) instead of all bricks
+    const uint32_t* voxList;
+    int64_t nVoxList;
+    // optional per-query caps (squared) on the search radius, indexed like the outputs; with caps,
+    // a query that found nothing below its (smaller than the plain limit) cap is not written but
+    // appended to rerunVox / rerunPts, counted in rerunCount[0] / [1]
+    const float* capSdf;
+    const float* capProj;
+    uint32_t* rerunVox;
+    uint32_t* rerunPts;
+    uint32_t* rerunCount;
 };
 
 // persistent launch over items[0..nItems) (device array); tickets = 8 zeroed 64-bit counters;

@@ -7,6 +7,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -231,13 +232,12 @@ int mi_proj_find_projections_learned_device(mi_mesh* mesh, mi_mlp* m, float marg
         return mi_internal_fail(MI_E_INVALID, "invalid arguments");
     if (reruns) *reruns = 0;
     if (n == 0) return MI_OK;
-    {
-        std::lock_guard<std::mutex> lk(m->mu);
-        MLP_HIP(hipSetDevice(m->device));
-        MLP_HIP(m->work.ensure((size_t)n * 4));
-        MLP_HIP(mi::mlp::forward(m->params, d_pts, nullptr, n, m->work.as<float>(), margin,
-                                 static_cast<hipStream_t>(stream)));
-    }
+    // the network's lock is held through the capped search, which reads the caps from its workspace
+    std::lock_guard<std::mutex> lk(m->mu);
+    MLP_HIP(hipSetDevice(m->device));
+    MLP_HIP(m->work.ensure((size_t)n * 4));
+    MLP_HIP(mi::mlp::forward(m->params, d_pts, nullptr, n, m->work.as<float>(), margin,
+                             static_cast<hipStream_t>(stream)));
     return mi_proj_find_projections_capped_device(mesh, d_pts, n, m->work.as<float>(), upDistLimitSq, d_res, reruns,
                                                   stream);
 }
@@ -252,18 +252,70 @@ int mi_sdf_grid_learned_device(mi_mesh* mesh, mi_mlp* m, float margin, const int
     if (reruns) *reruns = 0;
     const size_t nvox = (size_t)dims[0] * dims[1] * (size_t)(zEnd - zBegin);
     if (nvox == 0) return MI_OK;
-    {
-        std::lock_guard<std::mutex> lk(m->mu);
-        MLP_HIP(hipSetDevice(m->device));
-        const size_t capBytes = align256(nvox * 4);
-        MLP_HIP(m->work.ensure(capBytes + nvox * 12));
-        float* caps = m->work.as<float>();
-        float* pts = reinterpret_cast<float*>(m->work.as<char>() + capBytes);
-        hipStream_t s = static_cast<hipStream_t>(stream);
-        MLP_HIP(mi::mlp::gridPoints(dims, xf, zBegin, zEnd, pts, s));
-        MLP_HIP(mi::mlp::forward(m->params, pts, nullptr, (int64_t)nvox, caps, margin, s));
-    }
+    // the network's lock is held through the capped search, which reads the caps from its workspace
+    std::lock_guard<std::mutex> lk(m->mu);
+    MLP_HIP(hipSetDevice(m->device));
+    const size_t capBytes = align256(nvox * 4);
+    MLP_HIP(m->work.ensure(capBytes + nvox * 12));
+    float* caps = m->work.as<float>();
+    float* pts = reinterpret_cast<float*>(m->work.as<char>() + capBytes);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    MLP_HIP(mi::mlp::gridPoints(dims, xf, zBegin, zEnd, pts, s));
+    MLP_HIP(mi::mlp::forward(m->params, pts, nullptr, (int64_t)nvox, caps, margin, s));
     return mi_sdf_grid_capped_device(mesh, dims, xf, opts, m->work.as<float>(), zBegin, zEnd, d_res, reruns, stream);
+}
+
+int mi_batch_run_learned_device(mi_batch* b, const mi_batch_item* items, int32_t count, mi_mlp* const* nets,
+                                float margin, const mi_dist_options* opts, float upDistLimitSq, void* stream,
+                                int64_t* reruns)
+{
+    if (!b || count < 0 || (count > 0 && (!items || !nets)) || !opts || !(margin >= 0))
+        return mi_internal_fail(MI_E_INVALID, "invalid arguments");
+    if (reruns) *reruns = 0;
+    if (count == 0) return MI_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    // caps of every item (squared, from its own network), then the capped two-pass batch
+    size_t need = 0, gridTmp = 0;
+    for (int32_t i = 0; i < count; ++i) {
+        const mi_batch_item& it = items[i];
+        if (it.dims[0] < 0 || it.dims[1] < 0 || it.dims[2] < 0 || it.numPts < 0)
+            return mi_internal_fail(MI_E_INVALID, "invalid batch item");
+        const size_t nvox = (size_t)it.dims[0] * it.dims[1] * it.dims[2];
+        need += align256(nvox * 4) + align256((size_t)it.numPts * 4);
+        gridTmp = std::max(gridTmp, nvox * 12);
+    }
+    Buf caps;
+    MLP_HIP(caps.ensure(align256(gridTmp) + need + 256));
+    float* tmp = caps.as<float>();
+    char* base = caps.as<char>() + align256(gridTmp);
+    std::vector<const float*> capSdf((size_t)count, nullptr), capProj((size_t)count, nullptr);
+    size_t off = 0;
+    for (int32_t i = 0; i < count; ++i) {
+        const mi_batch_item& it = items[i];
+        const size_t nvox = (size_t)it.dims[0] * it.dims[1] * it.dims[2];
+        float* cs = reinterpret_cast<float*>(base + off);
+        off += align256(nvox * 4);
+        float* cp = reinterpret_cast<float*>(base + off);
+        off += align256((size_t)it.numPts * 4);
+        mi_mlp* net = nets[i];
+        if (!net) continue;
+        std::lock_guard<std::mutex> lk(net->mu);
+        MLP_HIP(hipSetDevice(net->device));
+        if (nvox) {
+            MLP_HIP(mi::mlp::gridPoints(it.dims, it.xf, 0, it.dims[2], tmp, s));
+            MLP_HIP(mi::mlp::forward(net->params, tmp, nullptr, (int64_t)nvox, cs, margin, s));
+            capSdf[i] = cs;
+        }
+        if (it.numPts) {
+            MLP_HIP(mi::mlp::forward(net->params, it.d_pts, nullptr, it.numPts, cp, margin, s));
+            capProj[i] = cp;
+        }
+    }
+    const int rc = mi_batch_run_capped_device(b, items, count, capSdf.data(), capProj.data(), opts, upDistLimitSq,
+                                              stream, reruns);
+    // the caps are read by the launches queued above: keep them until those have run
+    (void)hipStreamSynchronize(s);
+    return rc;
 }
 
 } // extern "C"

@@ -60,3 +60,12 @@ def test_host_tree_depth_bound():
     nodes, dips, depth = M.build_tree_host(v, f)
     assert depth == int(np.ceil(np.log2(f.shape[0])))
     assert M.lib().mi_proj_heap_bytes(1000) >= 1000 * 44
+
+
+def test_tree_builder_setting():
+    assert M.get_tree_builder() == "device"
+    M.set_tree_builder("host")
+    assert M.get_tree_builder() == "host"
+    M.set_tree_builder("device")
+    assert M.lib().mi_set_tree_builder(7) == M.MI_E_INVALID
+    assert M.get_tree_builder() == "device"
