@@ -35,9 +35,9 @@ def log(*a):
 # over `bench.py --steps 1 --warmup 0 --no-cpu --no-learned --no-plain --copy-calibration`
 # (profiles/sdf_grid_coop_k8_pmc.md): FETCH_SIZE and WRITE_SIZE in KiB per dispatch, each scaled
 # by bytes / counter reading of the 1 GiB device copy in the same run.
-PMC_SDF_KIB = {"fetch": 20467738.0, "write": 152168.0}
-PMC_COPY_KIB = {"fetch": 524318.0, "write": 1048580.0}
-PMC_SDF_L2_HIT = 602191000.0 / (602191000.0 + 182081000.0)    # TCC_HIT_sum / (HIT + MISS)
+PMC_SDF_KIB = {"fetch": 20453141.9, "write": 151558.2}
+PMC_COPY_KIB = {"fetch": 524308.4, "write": 1048576.0}
+PMC_SDF_L2_HIT = 600924019.0 / (600924019.0 + 182379804.0)    # TCC_HIT_sum / (HIT + MISS)
 COPY_BYTES = 1 << 30
 
 

@@ -422,6 +422,31 @@ __device__ __forceinline__ float laneValue(float v, int lane)
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
+// Upper bound of one query's squared distance: a greedy descent (nearer child box at every level,
+// per lane) to one leaf, then that leaf's exact distance (the leaf test of the walks).  It is a real
+// candidate, so seeding a walk with it leaves the result unchanged while every lane prunes from the
+// first node on -- the walks of a wave whose lanes are far apart no longer follow one lane's order
+// with the other lanes' limits still open.
+__device__ __forceinline__ float greedyLeafDistSq(const MeshDev& m, float px, float py, float pz, const double pd[3])
+{
+    int cur = m.rootRef;
+    while (cur >= 0) {
+        const float4 n0 = m.dnode[4 * cur], n1 = m.dnode[4 * cur + 1];
+        const float4 n2 = m.dnode[4 * cur + 2], n3 = m.dnode[4 * cur + 3];
+        const float d1 = boxDistSq(n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, px, py, pz);
+        const float d2 = boxDistSq(n1.z, n1.w, n2.x, n2.y, n2.z, n2.w, px, py, pz);
+        cur = d1 <= d2 ? __float_as_int(n3.x) : __float_as_int(n3.y);
+    }
+    const Leaf L = loadLeaf(m.tri, ~cur);
+    const double ad[3] = { L.a[0], L.a[1], L.a[2] };
+    const double bd[3] = { L.b[0], L.b[1], L.b[2] };
+    const double cd[3] = { L.c[0], L.c[1], L.c[2] };
+    double pr[3], ba, bb;
+    closestPointD(pd, ad, bd, cd, pr, ba, bb);
+    const float dx = (float)pr[0] - px, dy = (float)pr[1] - py, dz = (float)pr[2] - pz;
+    return dx * dx + dy * dy + dz * dz;
+}
+
 // Minimum squared distance of each lane's point to the mesh below `up` (every leaf whose box is
 // closer than the lane's current best is tested, as in findProjectionSubtree).  The result is a
 // minimum over candidates, so it does not depend on the visit order; lanes pass up < 0 to opt out.
@@ -432,6 +457,10 @@ __device__ __forceinline__ float coopDistance(const MeshDev& m, float px, float 
     float best = up;
     if (m.numFaces <= 0) return best;
     const double pd[3] = { px, py, pz };
+    if (up > 0) {
+        const float g = greedyLeafDistSq(m, px, py, pz, pd);
+        if (g < best) best = g;
+    }
     int sp = 0;
     int cur = m.rootRef;
     float curd = boxDistSq(m.rootMin[0], m.rootMin[1], m.rootMin[2], m.rootMax[0], m.rootMax[1],
@@ -566,6 +595,16 @@ __device__ __forceinline__ void coopProject(const MeshDev& m, float px, float py
     uint32_t bestKey = 0;
     if (m.numFaces <= 0) return;
     const double pd[3] = { px, py, pz };
+    // search radius just above the greedy leaf's distance: that leaf (and every leaf at the minimum)
+    // is still accepted, so the record is unchanged; an empty search keeps the caller's limit
+    if (valid && up > 0) {
+        const float g = greedyLeafDistSq(m, px, py, pz, pd);
+        if (g < up) {
+            // any radius above g keeps that leaf; at least FLT_MIN so no comparison meets a denormal
+            const float above = g < FLT_MIN ? FLT_MIN : __uint_as_float(__float_as_uint(g) + 1u);
+            res.distSq = above < up ? above : up;
+        }
+    }
     int sp = 0;
     int cur = m.rootRef, curDepth = 0;
     float curd = boxDistSq(m.rootMin[0], m.rootMin[1], m.rootMin[2], m.rootMax[0], m.rootMax[1],
