@@ -234,14 +234,14 @@ int mi_get_tree_builder(void) { return g_treeBuilder.load(); }
 namespace {
 
 struct ArenaLayout {
-    size_t dnode, tri, dint, dleaf;
-    size_t total() const { return dnode + tri + dint + dleaf; }
+    size_t dnode, tri, dint, dleaf, dquad;
+    size_t total() const { return dnode + tri + dint + dleaf + dquad; }
 };
 
 ArenaLayout arenaLayout(int32_t nf)
 {
     const size_t ni = nf > 0 ? (size_t)nf - 1 : 0, nl = nf > 0 ? (size_t)nf : 0;
-    return { align256(ni * 64), align256(nl * 48), align256(ni * 32), align256(nl * 32) };
+    return { align256(ni * 64), align256(nl * 48), align256(ni * 32), align256(nl * 32), align256(ni * 128) };
 }
 
 void bindArena(mi_mesh* m, const ArenaLayout& L)
@@ -251,6 +251,18 @@ void bindArena(mi_mesh* m, const ArenaLayout& L)
     m->dev.tri = reinterpret_cast<const float4*>(base + L.dnode);
     m->dev.dint = reinterpret_cast<const float4*>(base + L.dnode + L.tri);
     m->dev.dleaf = reinterpret_cast<const float4*>(base + L.dnode + L.tri + L.dint);
+    m->dev.dquad = reinterpret_cast<const float4*>(base + L.dnode + L.tri + L.dint + L.dleaf);
+}
+
+// 4-wide records of the cooperative walks, from the binary records already in the arena
+int finishArena(mi_mesh* m, const ArenaLayout& L)
+{
+    if (m->nf < 2) return MI_OK;
+    char* base = m->arena.as<char>();
+    MI_HIP(mi::buildQuads(m->dev.dnode, m->nf - 1,
+                          reinterpret_cast<float4*>(base + L.dnode + L.tri + L.dint + L.dleaf), m->stream));
+    MI_HIP(hipStreamSynchronize(m->stream));
+    return MI_OK;
 }
 
 void setRoot(mi_mesh* m, const mi::Box3& rootBox, int32_t rootRef)
@@ -292,6 +304,7 @@ int createHostBuilt(mi_mesh* m, const float* verts, const int32_t* tris)
             MI_HIP(hipMemcpy(base + L.dnode + L.tri, ft.dint.data(), ft.dint.size() * 4, hipMemcpyHostToDevice));
         MI_HIP(hipMemcpy(base + L.dnode + L.tri + L.dint, ft.dleaf.data(), ft.dleaf.size() * 4,
                          hipMemcpyHostToDevice));
+        if (int rc = finishArena(m, L)) return rc;
     }
     if (m->nf > 0) setRoot(m, ft.rootBox, ft.rootRef);
     return MI_OK;
@@ -328,6 +341,7 @@ int createDeviceBuilt(mi_mesh* m, const float* verts, const int32_t* tris)
     m->depth = info.depth;
     m->stackD = mi::stackDepthFor(m->depth);
     if (m->stackD == 0) return fail(MI_E_LIMIT, "tree deeper than 32 levels");
+    if (int rc = finishArena(m, L)) return rc;
     setRoot(m, info.rootBox, info.rootRef);
     return MI_OK;
 }

@@ -29,6 +29,10 @@ namespace mi {
 namespace {
 
 constexpr int kBlock = 256;          // 4 waves
+
+// pending entries of a 4-wide walk over a tree of depth <= D: each step pushes at most 3 and goes
+// down two levels (one only into a leaf, where the path ends)
+constexpr int quadStack(int D) { return 3 * (D / 2 + 1); }
 constexpr int kXcds = 8;
 constexpr float kInv4Pi = 1.0f / (4 * 3.14159265358979f);   // INV_4PI, MRDipole.cpp:125
 
@@ -427,8 +431,9 @@ __device__ __forceinline__ float laneValue(float v, int lane)
 // candidate, so seeding a walk with it leaves the result unchanged while every lane prunes from the
 // first node on -- the walks of a wave whose lanes are far apart no longer follow one lane's order
 // with the other lanes' limits still open.
-__device__ __forceinline__ float greedyLeafDistSq(const MeshDev& m, float px, float py, float pz, const double pd[3])
+__device__ __forceinline__ float greedyLeafDistSq(const MeshDev& m, float px, float py, float pz)
 {
+    const double pd[3] = { px, py, pz };
     int cur = m.rootRef;
     while (cur >= 0) {
         const float4 n0 = m.dnode[4 * cur], n1 = m.dnode[4 * cur + 1];
@@ -450,15 +455,69 @@ __device__ __forceinline__ float greedyLeafDistSq(const MeshDev& m, float px, fl
 // Minimum squared distance of each lane's point to the mesh below `up` (every leaf whose box is
 // closer than the lane's current best is tested, as in findProjectionSubtree).  The result is a
 // minimum over candidates, so it does not depend on the visit order; lanes pass up < 0 to opt out.
+// Slots of a 4-wide record q[0..7] (built by kBuildQuads): slot i's box and ref.
+struct Quad {
+    float mn[4][3], mx[4][3];
+    int32_t ref[4];
+    bool leafA, leafB;      // left / right child is a leaf (it fills slot 0 / 2, slot 1 / 3 empty)
+};
+
+__device__ __forceinline__ Quad loadQuad(const MeshDev& m, int j)
+{
+    const float4* q = m.dquad + 8 * (size_t)j;
+    const float4 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3], q4 = q[4], q5 = q[5], q6 = q[6], q7 = q[7];
+    Quad Q;
+    Q.mn[0][0] = q0.x; Q.mn[0][1] = q0.y; Q.mn[0][2] = q0.z; Q.mx[0][0] = q0.w; Q.mx[0][1] = q1.x; Q.mx[0][2] = q1.y;
+    Q.mn[1][0] = q1.z; Q.mn[1][1] = q1.w; Q.mn[1][2] = q2.x; Q.mx[1][0] = q2.y; Q.mx[1][1] = q2.z; Q.mx[1][2] = q2.w;
+    Q.mn[2][0] = q3.x; Q.mn[2][1] = q3.y; Q.mn[2][2] = q3.z; Q.mx[2][0] = q3.w; Q.mx[2][1] = q4.x; Q.mx[2][2] = q4.y;
+    Q.mn[3][0] = q4.z; Q.mn[3][1] = q4.w; Q.mn[3][2] = q5.x; Q.mx[3][0] = q5.y; Q.mx[3][1] = q5.z; Q.mx[3][2] = q5.w;
+    Q.ref[0] = __float_as_int(q6.x); Q.ref[1] = __float_as_int(q6.y);
+    Q.ref[2] = __float_as_int(q6.z); Q.ref[3] = __float_as_int(q6.w);
+    Q.leafA = __float_as_int(q7.x) != 0;
+    Q.leafB = __float_as_int(q7.y) != 0;
+    return Q;
+}
+
+__device__ __forceinline__ float slotDistSq(const Quad& Q, int i, float px, float py, float pz)
+{
+    return boxDistSq(Q.mn[i][0], Q.mn[i][1], Q.mn[i][2], Q.mx[i][0], Q.mx[i][1], Q.mx[i][2], px, py, pz);
+}
+
+// Ranks of the active slots (b[i] != 0) by the lead lane's distance, ties by slot index: 0 = the
+// slot the walk continues with, the others go on the stack farthest first.
+__device__ __forceinline__ int rankSlots(const uint64_t b[4], const float d[4], int rank[4])
+{
+    const uint64_t any = b[0] | b[1] | b[2] | b[3];
+    const int lead = __builtin_ctzll(any);
+    uint32_t k[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) k[i] = b[i] ? __float_as_uint(laneValue(d[i], lead)) : 0xffffffffu;
+    int cnt = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        cnt += b[i] != 0;
+        int r = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (j != i && b[j] && (k[j] < k[i] || (k[j] == k[i] && j < i))) ++r;
+        rank[i] = r;
+    }
+    return cnt;
+}
+
+// Squared distance to the nearest leaf (calcDistanceSq / findProjectionSubtree distance part,
+// source/MRCuda/MRCudaFastWindingNumber.cu:65-133) for each lane, on a wave-shared walk over the
+// 4-wide records: the minimum does not depend on the visit order, so it equals the per-lane
+// result bit for bit.  A step tests the four grandchildren of a node; the nearest active one (for
+// the lowest active lane) continues, the other active ones are pushed farthest first.
 template <int D>
 __device__ __forceinline__ float coopDistance(const MeshDev& m, float px, float py, float pz, float up,
                                               float4* stk)
 {
     float best = up;
     if (m.numFaces <= 0) return best;
-    const double pd[3] = { px, py, pz };
     if (up > 0) {
-        const float g = greedyLeafDistSq(m, px, py, pz, pd);
+        const float g = greedyLeafDistSq(m, px, py, pz);
         if (g < best) best = g;
     }
     int sp = 0;
@@ -481,6 +540,7 @@ __device__ __forceinline__ float coopDistance(const MeshDev& m, float px, float 
         if (cur < 0) {
             const Leaf L = loadLeaf(m.tri, ~cur);
             if (act) {
+                const double pd[3] = { px, py, pz };
                 const double ad[3] = { L.a[0], L.a[1], L.a[2] };
                 const double bd[3] = { L.b[0], L.b[1], L.b[2] };
                 const double cd[3] = { L.c[0], L.c[1], L.c[2] };
@@ -492,33 +552,31 @@ __device__ __forceinline__ float coopDistance(const MeshDev& m, float px, float 
             }
             continue;
         }
-        const float4 n0 = m.dnode[4 * cur], n1 = m.dnode[4 * cur + 1];
-        const float4 n2 = m.dnode[4 * cur + 2], n3 = m.dnode[4 * cur + 3];
-        const float d1 = boxDistSq(n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, px, py, pz);
-        const float d2 = boxDistSq(n1.z, n1.w, n2.x, n2.y, n2.z, n2.w, px, py, pz);
-        const uint64_t b1 = waveBallot(act && d1 < best), b2 = waveBallot(act && d2 < best);
-        if (!(b1 | b2)) continue;
-        bool leftFirst = b1 != 0;
-        if (b1 && b2) {
-            const int lead = __builtin_ctzll(b1 | b2);
-            leftFirst = laneValue(d1, lead) < laneValue(d2, lead);
+        const Quad Q = loadQuad(m, cur);
+        float d[4];
+        uint64_t b[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            d[i] = slotDistSq(Q, i, px, py, pz);
+            b[i] = waveBallot(act && d[i] < best);
         }
-        // the later child goes on the stack, the earlier one stays current
-        if (leftFirst) {
-            if (b2) {
-                stk[2 * sp] = make_float4(n3.y, n1.z, n1.w, n2.x);
-                stk[2 * sp + 1] = make_float4(n2.y, n2.z, n2.w, 0.f);
-                ++sp;
+        if (!(b[0] | b[1] | b[2] | b[3])) continue;
+        int rank[4];
+        const int cnt = rankSlots(b, d, rank);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            if (!b[i]) continue;
+            if (rank[i] == 0) {
+                cur = Q.ref[i];
+                curd = d[i];
+            } else {
+                const int pos = sp + cnt - 1 - rank[i];
+                stk[2 * pos] = make_float4(__int_as_float(Q.ref[i]), Q.mn[i][0], Q.mn[i][1], Q.mn[i][2]);
+                stk[2 * pos + 1] = make_float4(Q.mx[i][0], Q.mx[i][1], Q.mx[i][2], 0.f);
             }
-            cur = __float_as_int(n3.x); curd = d1; have = true;
-        } else {
-            if (b1) {
-                stk[2 * sp] = make_float4(n3.x, n0.x, n0.y, n0.z);
-                stk[2 * sp + 1] = make_float4(n0.w, n1.x, n1.y, 0.f);
-                ++sp;
-            }
-            cur = __float_as_int(n3.y); curd = d2; have = true;
         }
+        sp += cnt - 1;
+        have = true;
     }
     return best;
 }
@@ -573,17 +631,20 @@ __device__ __forceinline__ float coopWinding(const MeshDev& m, float qx, float q
 }
 
 // Closest point for each lane (findProjectionSubtree, source/MRMesh/MRMeshProject.cpp:13-103) on a
-// wave-shared walk.  The reference's answer is the leaf that minimises (distSq, k) where k is the
-// leaf's position in that lane's own depth-first order (nearer child first, ties to the right):
-// candidates are accepted only on a strict '<' and, while the best is still above the minimum,
-// no ancestor of the first minimal leaf can be pruned.  k is the lane's path of near(0)/far(1)
-// choices, MSB first (bit 31 - depth for a child one level below `depth`), so comparing keys
-// compares positions in the lane's order.  Pruning and acceptance both compare (distSq, key)
-// lexicographically, which makes the record bit-identical to the reference for any visit order.
+// wave-shared walk over the 4-wide records.  The reference's answer is the leaf that minimises
+// (distSq, k) where k is the leaf's position in that lane's own depth-first order (nearer child
+// first, ties to the right): candidates are accepted only on a strict '<' and, while the best is
+// still above the minimum, no ancestor of the first minimal leaf can be pruned.  k is the lane's
+// path of near(0)/far(1) choices, MSB first (bit 31 - depth for a child one level below `depth`),
+// so comparing keys compares positions in the lane's order.  Pruning and acceptance both compare
+// (distSq, key) lexicographically, which makes the record bit-identical to the reference for any
+// visit order.  A slot passes only if its parent would have passed, so the 4-wide step never
+// visits what the binary walk would have pruned.
 // Valid when loDistLimitSq <= 0 (the reference's early exit then only fires at distSq == 0,
 // which is the minimum) and for trees of depth <= 32.
-// No per-lane key is stored on the stack: a pending entry is always the sibling of the current
-// path's node at the entry's depth d, so its key is the current key cut below d with bit d flipped.
+// Keys are not stored per lane: a popped entry is related to the slot popped or continued just
+// before it in the same step -- its sibling (flip the entry's bit, clear below), or a child of
+// the other child (flip the bit one level up, then the entry's own bit from a 64-lane mask).
 template <int D>
 __device__ __forceinline__ void coopProject(const MeshDev& m, float px, float py, float pz, float up, bool valid,
                                             float4* stk, BestHit& res)
@@ -594,17 +655,17 @@ __device__ __forceinline__ void coopProject(const MeshDev& m, float px, float py
     res.a = res.b = 0;
     uint32_t bestKey = 0;
     if (m.numFaces <= 0) return;
-    const double pd[3] = { px, py, pz };
     // search radius just above the greedy leaf's distance: that leaf (and every leaf at the minimum)
     // is still accepted, so the record is unchanged; an empty search keeps the caller's limit
     if (valid && up > 0) {
-        const float g = greedyLeafDistSq(m, px, py, pz, pd);
+        const float g = greedyLeafDistSq(m, px, py, pz);
         if (g < up) {
             // any radius above g keeps that leaf; at least FLT_MIN so no comparison meets a denormal
             const float above = g < FLT_MIN ? FLT_MIN : __uint_as_float(__float_as_uint(g) + 1u);
             res.distSq = above < up ? above : up;
         }
     }
+    const int lane = threadIdx.x & 63;
     int sp = 0;
     int cur = m.rootRef, curDepth = 0;
     float curd = boxDistSq(m.rootMin[0], m.rootMin[1], m.rootMin[2], m.rootMax[0], m.rootMax[1],
@@ -615,12 +676,20 @@ __device__ __forceinline__ void coopProject(const MeshDev& m, float px, float py
         if (!have) {
             if (sp == 0) break;
             --sp;
-            const float4 e0 = stk[2 * sp], e1 = stk[2 * sp + 1];
+            const float4 e0 = stk[3 * sp], e1 = stk[3 * sp + 1], e2 = stk[3 * sp + 2];
             cur = __float_as_int(e0.x);
-            curDepth = __float_as_int(e1.w);          // >= 1
             curd = boxDistSq(e0.y, e0.z, e0.w, e1.x, e1.y, e1.z, px, py, pz);
+            const int info = __float_as_int(e1.w);
+            curDepth = info & 0xff;                        // >= 1
             const uint32_t bit = 1u << (32 - curDepth);
-            curKey = (curKey & ~(bit - 1u)) ^ bit;
+            if (info & 0x100) {
+                const uint32_t pbit = bit << 1;
+                curKey = (curKey & ~(pbit - 1u)) ^ pbit;
+                const uint64_t mask = (uint64_t)__float_as_uint(e2.x) | ((uint64_t)__float_as_uint(e2.y) << 32);
+                if ((mask >> lane) & 1ull) curKey |= bit;
+            } else {
+                curKey = (curKey & ~(bit - 1u)) ^ bit;
+            }
         }
         have = false;
         const bool act = curd < res.distSq || (curd == res.distSq && curKey < bestKey);
@@ -630,6 +699,7 @@ __device__ __forceinline__ void coopProject(const MeshDev& m, float px, float py
         if (cur < 0) {
             const Leaf L = loadLeaf(m.tri, ~cur);
             if (act) {
+                const double pd[3] = { px, py, pz };
                 const double ad[3] = { L.a[0], L.a[1], L.a[2] };
                 const double bd[3] = { L.b[0], L.b[1], L.b[2] };
                 const double cd[3] = { L.c[0], L.c[1], L.c[2] };
@@ -648,39 +718,67 @@ __device__ __forceinline__ void coopProject(const MeshDev& m, float px, float py
             }
             continue;
         }
-        const float4 n0 = m.dnode[4 * cur], n1 = m.dnode[4 * cur + 1];
-        const float4 n2 = m.dnode[4 * cur + 2], n3 = m.dnode[4 * cur + 3];
-        const float d1 = boxDistSq(n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, px, py, pz);
-        const float d2 = boxDistSq(n1.z, n1.w, n2.x, n2.y, n2.z, n2.w, px, py, pz);
-        const uint32_t bit = 1u << (31 - curDepth);
-        const bool leftNear = d1 < d2;                       // ties: right child first
-        const uint32_t kL = curKey | (leftNear ? 0u : bit), kR = curKey | (leftNear ? bit : 0u);
-        const bool aL = act && (d1 < res.distSq || (d1 == res.distSq && kL < bestKey));
-        const bool aR = act && (d2 < res.distSq || (d2 == res.distSq && kR < bestKey));
-        const uint64_t b1 = waveBallot(aL), b2 = waveBallot(aR);
-        if (!(b1 | b2)) continue;
-        bool leftFirst = b1 != 0;
-        if (b1 && b2) {
-            const int lead = __builtin_ctzll(b1 | b2);
-            leftFirst = laneValue(d1, lead) < laneValue(d2, lead);
-        }
-        const int childDepth = curDepth + 1;
-        if (leftFirst) {
-            if (b2) {
-                stk[2 * sp] = make_float4(n3.y, n1.z, n1.w, n2.x);
-                stk[2 * sp + 1] = make_float4(n2.y, n2.z, n2.w, __int_as_float(childDepth));
-                ++sp;
+        const Quad Q = loadQuad(m, cur);
+        float d[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) d[i] = slotDistSq(Q, i, px, py, pz);
+        // the two children (union of their slots; an empty slot adds nothing) decide this lane's
+        // first choice, the grandchildren its second
+        const float dA = boxDistSq(fminf(Q.mn[0][0], Q.mn[1][0]), fminf(Q.mn[0][1], Q.mn[1][1]),
+                                   fminf(Q.mn[0][2], Q.mn[1][2]), fmaxf(Q.mx[0][0], Q.mx[1][0]),
+                                   fmaxf(Q.mx[0][1], Q.mx[1][1]), fmaxf(Q.mx[0][2], Q.mx[1][2]), px, py, pz);
+        const float dB = boxDistSq(fminf(Q.mn[2][0], Q.mn[3][0]), fminf(Q.mn[2][1], Q.mn[3][1]),
+                                   fminf(Q.mn[2][2], Q.mn[3][2]), fmaxf(Q.mx[2][0], Q.mx[3][0]),
+                                   fmaxf(Q.mx[2][1], Q.mx[3][1]), fmaxf(Q.mx[2][2], Q.mx[3][2]), px, py, pz);
+        const bool nearA = dA < dB;                           // ties: right child first
+        const uint32_t bit1 = 1u << (31 - curDepth), bit2 = bit1 >> 1;
+        const uint32_t kA = curKey | (nearA ? 0u : bit1), kB = curKey | (nearA ? bit1 : 0u);
+        const bool nearA0 = d[0] < d[1], nearB0 = d[2] < d[3];
+        uint32_t key[4];
+        key[0] = Q.leafA ? kA : (kA | (nearA0 ? 0u : bit2));
+        key[1] = kA | (nearA0 ? bit2 : 0u);
+        key[2] = Q.leafB ? kB : (kB | (nearB0 ? 0u : bit2));
+        key[3] = kB | (nearB0 ? bit2 : 0u);
+        uint64_t b[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            b[i] = waveBallot(act && (d[i] < res.distSq || (d[i] == res.distSq && key[i] < bestKey)));
+        if (!(b[0] | b[1] | b[2] | b[3])) continue;
+        int rank[4];
+        const int cnt = rankSlots(b, d, rank);
+        const int baseDepth = curDepth;
+        // side (0: under the left child, 1: right) of the slot at each rank
+        uint32_t sideAt = 0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            if (b[i]) sideAt |= (uint32_t)(i >> 1) << rank[i];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            if (!b[i]) continue;
+            const int side = i >> 1;
+            const bool leafSide = side ? Q.leafB : Q.leafA;
+            const int depthI = baseDepth + (leafSide ? 1 : 2);
+            if (rank[i] == 0) {
+                cur = Q.ref[i];
+                curd = d[i];
+                curKey = key[i];
+                curDepth = depthI;
+                continue;
             }
-            cur = __float_as_int(n3.x); curd = d1; curKey = kL;
-        } else {
-            if (b1) {
-                stk[2 * sp] = make_float4(n3.x, n0.x, n0.y, n0.z);
-                stk[2 * sp + 1] = make_float4(n0.w, n1.x, n1.y, __int_as_float(childDepth));
-                ++sp;
+            // relation to the slot handled just before this one
+            const int prevSide = (int)((sideAt >> (rank[i] - 1)) & 1u);
+            int info = depthI;
+            uint64_t mask = 0;
+            if (prevSide != side && !leafSide) {
+                info |= 0x100;
+                mask = waveBallot((key[i] & bit2) != 0u);
             }
-            cur = __float_as_int(n3.y); curd = d2; curKey = kR;
+            const int pos = sp + cnt - 1 - rank[i];
+            stk[3 * pos] = make_float4(__int_as_float(Q.ref[i]), Q.mn[i][0], Q.mn[i][1], Q.mn[i][2]);
+            stk[3 * pos + 1] = make_float4(Q.mx[i][0], Q.mx[i][1], Q.mx[i][2], __int_as_float(info));
+            stk[3 * pos + 2] = make_float4(__uint_as_float((uint32_t)mask), __uint_as_float((uint32_t)(mask >> 32)), 0.f, 0.f);
         }
-        curDepth = childDepth;
+        sp += cnt - 1;
         have = true;
     }
 }
@@ -703,7 +801,7 @@ __device__ __forceinline__ void appendWave(bool p, uint32_t value, uint32_t* lis
 // appended to the list, to be searched again without the cap (any cap above the true distance
 // gives the reference's record, so the combination is exact).
 template <int D>
-__global__ __launch_bounds__(kBlock) void kProjectionsCoop(MeshDev m, const float* __restrict__ pts,
+__global__ __launch_bounds__(kBlock, 7) void kProjectionsCoop(MeshDev m, const float* __restrict__ pts,
                                                            const uint32_t* __restrict__ perm, int64_t n,
                                                            int nLogical, Xf ptXf, int hasPtXf, float up,
                                                            const float* __restrict__ bound,
@@ -711,7 +809,7 @@ __global__ __launch_bounds__(kBlock) void kProjectionsCoop(MeshDev m, const floa
                                                            uint32_t* __restrict__ rerunCount,
                                                            ProjRec* __restrict__ out)
 {
-    __shared__ float4 stkMem[kWaves][2 * D];
+    __shared__ float4 stkMem[kWaves][3 * quadStack(D)];
     const int lb = logicalBlock(nLogical);
     if (lb >= nLogical) return;
     const int64_t i = (int64_t)lb * kBlock + threadIdx.x;
@@ -753,14 +851,14 @@ __device__ __forceinline__ void sdfStore(const MeshDev& m, float px, float py, f
 }
 
 template <int D>
-__global__ __launch_bounds__(kBlock) void kSdfGridCoop(MeshDev m, int dx, int dy, int zBegin, int zEnd,
+__global__ __launch_bounds__(kBlock, 8) void kSdfGridCoop(MeshDev m, int dx, int dy, int zBegin, int zEnd,
                                                        int nbx, int nby, int nbz, int nLogical, Xf xf,
                                                        DistOpts o, const float* __restrict__ bound,
                                                        uint32_t* __restrict__ rerun,
                                                        uint32_t* __restrict__ rerunCount,
                                                        float* __restrict__ out)
 {
-    __shared__ float4 stkMem[kWaves][2 * D];
+    __shared__ float4 stkMem[kWaves][2 * quadStack(D)];
     const int lb = logicalBlock(nLogical);
     if (lb >= nLogical) return;
     const int lane = threadIdx.x & 63;
@@ -782,12 +880,12 @@ __global__ __launch_bounds__(kBlock) void kSdfGridCoop(MeshDev m, int dx, int dy
 
 // the listed voxels of a z-slab (ids relative to the slab), searched without any cap
 template <int D>
-__global__ __launch_bounds__(kBlock) void kSdfVoxelsCoop(MeshDev m, int dx, int dy, int zBegin,
+__global__ __launch_bounds__(kBlock, 8) void kSdfVoxelsCoop(MeshDev m, int dx, int dy, int zBegin,
                                                          const uint32_t* __restrict__ list, int64_t count,
                                                          int nLogical, Xf xf, DistOpts o,
                                                          float* __restrict__ out)
 {
-    __shared__ float4 stkMem[kWaves][2 * D];
+    __shared__ float4 stkMem[kWaves][2 * quadStack(D)];
     const int lb = logicalBlock(nLogical);
     if (lb >= nLogical) return;
     const int64_t i = (int64_t)lb * kBlock + threadIdx.x;
@@ -872,11 +970,11 @@ __global__ __launch_bounds__(kBlock) void kSelfIntersectionsCoop(MeshDev m, int 
 // wave whose part is drained moves on to the next part: spatially adjacent units share an L2,
 // and the tail is balanced.  Every wave exits once it has seen every part drained.
 template <int D>
-__global__ __launch_bounds__(kBlock) void kBatch(const BatchItem* __restrict__ items, int nItems, int64_t total,
+__global__ __launch_bounds__(kBlock, 6) void kBatch(const BatchItem* __restrict__ items, int nItems, int64_t total,
                                                  DistOpts o, float up, unsigned long long* __restrict__ tickets,
                                                  int xcdSplit)
 {
-    __shared__ float4 stkMem[kWaves][2 * D];
+    __shared__ float4 stkMem[kWaves][3 * quadStack(D)];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int parts = xcdSplit ? kXcds : 1;
     int part = xcdSplit ? (int)(blockIdx.x % kXcds) : 0;
@@ -986,6 +1084,49 @@ hipError_t dispatchDepth(int D, Fn&& fn)
 } // namespace
 
 namespace {
+// 4-wide record of internal node j (layout read by loadQuad): slots 0,1 = the left child's
+// children, 2,3 = the right child's; a leaf child takes slot 0 / 2 itself and leaves slot 1 / 3
+// empty (inverted box: never closer than any limit)
+__global__ void kBuildQuads(const float4* __restrict__ dnode, int32_t ni, float4* __restrict__ dquad)
+{
+    const int32_t j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= ni) return;
+    const float4 n0 = dnode[4 * (size_t)j], n1 = dnode[4 * (size_t)j + 1];
+    const float4 n2 = dnode[4 * (size_t)j + 2], n3 = dnode[4 * (size_t)j + 3];
+    float bx[4][6];
+    int32_t ref[4];
+    int32_t leaf[2];
+    const float cb[2][6] = { { n0.x, n0.y, n0.z, n0.w, n1.x, n1.y }, { n1.z, n1.w, n2.x, n2.y, n2.z, n2.w } };
+    const int32_t child[2] = { __float_as_int(n3.x), __float_as_int(n3.y) };
+    for (int side = 0; side < 2; ++side) {
+        const int32_t c = child[side];
+        leaf[side] = c < 0 ? 1 : 0;
+        if (c >= 0) {
+            const float4 c0 = dnode[4 * (size_t)c], c1 = dnode[4 * (size_t)c + 1];
+            const float4 c2 = dnode[4 * (size_t)c + 2], c3 = dnode[4 * (size_t)c + 3];
+            const float l[6] = { c0.x, c0.y, c0.z, c0.w, c1.x, c1.y };
+            const float r[6] = { c1.z, c1.w, c2.x, c2.y, c2.z, c2.w };
+            for (int k = 0; k < 6; ++k) { bx[2 * side][k] = l[k]; bx[2 * side + 1][k] = r[k]; }
+            ref[2 * side] = __float_as_int(c3.x);
+            ref[2 * side + 1] = __float_as_int(c3.y);
+        } else {
+            for (int k = 0; k < 6; ++k) bx[2 * side][k] = cb[side][k];
+            for (int k = 0; k < 3; ++k) { bx[2 * side + 1][k] = INFINITY; bx[2 * side + 1][3 + k] = -INFINITY; }
+            ref[2 * side] = c;
+            ref[2 * side + 1] = 0;
+        }
+    }
+    float4* q = dquad + 8 * (size_t)j;
+    q[0] = make_float4(bx[0][0], bx[0][1], bx[0][2], bx[0][3]);
+    q[1] = make_float4(bx[0][4], bx[0][5], bx[1][0], bx[1][1]);
+    q[2] = make_float4(bx[1][2], bx[1][3], bx[1][4], bx[1][5]);
+    q[3] = make_float4(bx[2][0], bx[2][1], bx[2][2], bx[2][3]);
+    q[4] = make_float4(bx[2][4], bx[2][5], bx[3][0], bx[3][1]);
+    q[5] = make_float4(bx[3][2], bx[3][3], bx[3][4], bx[3][5]);
+    q[6] = make_float4(__int_as_float(ref[0]), __int_as_float(ref[1]), __int_as_float(ref[2]), __int_as_float(ref[3]));
+    q[7] = make_float4(__int_as_float(leaf[0]), __int_as_float(leaf[1]), 0.f, 0.f);
+}
+
 __global__ void kFillU32(uint32_t* p, int n, uint32_t v)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1004,6 +1145,13 @@ hipError_t launchBatch(const BatchItem* items, int nItems, int64_t totalUnits, c
                            xcdSplit);
         return hipGetLastError();
     });
+}
+
+hipError_t buildQuads(const float4* dnode, int32_t numInternal, float4* dquad, hipStream_t s)
+{
+    if (numInternal <= 0) return hipSuccess;
+    hipLaunchKernelGGL(kBuildQuads, dim3((numInternal + 255) / 256), dim3(256), 0, s, dnode, numInternal, dquad);
+    return hipGetLastError();
 }
 
 hipError_t fillU32(uint32_t* p, int n, uint32_t v, hipStream_t s)

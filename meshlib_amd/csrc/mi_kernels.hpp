@@ -13,6 +13,7 @@ struct MeshDev {
     const float4* tri;     // 3 per leaf slot
     const float4* dint;    // 2 per internal node
     const float4* dleaf;   // 2 per leaf slot
+    const float4* dquad;   // 8 per internal node: its grandchildren (4-wide records of the cooperative walks)
     float rootMin[3], rootMax[3];
     int32_t rootRef;
     int32_t numFaces;
@@ -62,6 +63,10 @@ hipError_t launchBatch(const BatchItem* items, int nItems, int64_t totalUnits, c
                        unsigned long long* tickets, int xcdSplit, int blocks, int stackD, hipStream_t s);
 
 // small fill kernel (used instead of hipMemsetAsync so streams stay graph-capture safe)
+// 4-wide records from the binary records: internal node j's four grandchildren, a leaf child
+// standing for itself in one slot with an empty partner slot
+hipError_t buildQuads(const float4* dnode, int32_t numInternal, float4* dquad, hipStream_t s);
+
 hipError_t fillU32(uint32_t* p, int n, uint32_t v, hipStream_t s);
 
 // stack depth template actually used for a tree of the given depth (0 if unsupported)
