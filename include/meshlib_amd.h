@@ -105,6 +105,15 @@ int mi_get_traversal(void);
 int mi_set_tree_builder(int builder);
 int mi_get_tree_builder(void);
 
+/* Step budget of the split form of the wave-cooperative signed-distance walk, process-wide (no
+ * counterpart in the reference).  A 64-voxel wave still walking after `steps` node visits hands its
+ * voxels, with their best candidates so far, to a second launch that walks the tree's 64 top
+ * subtrees for them in parallel; results are identical for every budget.  0 = never split.
+ * Default 4096.  The device entry points of one mesh share that mesh's workspace: calls on
+ * different streams must not overlap in time. */
+int     mi_set_split_budget(int32_t steps);
+int32_t mi_get_split_budget(void);
+
 /* ---------------------------------------------------------------- mesh data */
 
 /* Cuda::FastWindingNumber::prepareData_ (source/MRCuda/MRCudaFastWindingNumber.cpp:46-85)

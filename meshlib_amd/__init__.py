@@ -96,6 +96,8 @@ def lib():
         "mi_get_traversal": (C.c_int, []),
         "mi_set_tree_builder": (C.c_int, [C.c_int]),
         "mi_get_tree_builder": (C.c_int, []),
+        "mi_set_split_budget": (C.c_int, [I32]),
+        "mi_get_split_budget": (I32, []),
         "mi_mesh_create": (C.c_int, [P, I32, P, I32, C.POINTER(P)]),
         "mi_mesh_destroy": (None, [P]),
         "mi_mesh_num_nodes": (C.c_int, [P]),
@@ -206,6 +208,16 @@ def set_tree_builder(builder: str):
     """Where Mesh() builds the AABB tree and dipoles: 'device' (GPU, default) or 'host'.  Both give
     the reference's tree node-for-node and bit-for-bit."""
     _check(lib().mi_set_tree_builder(TREE_BUILDERS[builder]))
+
+
+def set_split_budget(steps: int):
+    """Steps after which a signed-distance wave hands its walk to the split pass (0 = never);
+    results do not depend on it."""
+    _check(lib().mi_set_split_budget(int(steps)))
+
+
+def get_split_budget() -> int:
+    return int(lib().mi_get_split_budget())
 
 
 def get_tree_builder() -> str:

@@ -24,6 +24,7 @@ namespace {
 thread_local std::string g_err;
 std::atomic<size_t> g_memLimit{ 0 };
 std::atomic<int> g_treeBuilder{ MI_TREE_BUILDER_DEVICE };
+std::atomic<int> g_splitBudget{ 4096 };
 
 int fail(int code, const std::string& msg)
 {
@@ -178,6 +179,36 @@ int ensureWorkspace(mi_mesh* m, size_t bytes)
     return MI_OK;
 }
 
+// scratch of the split form for grids of up to `layers` z-layers, in `buf` (grown if needed)
+int splitScratch(mi_mesh* m, DevBuf& buf, const int32_t dims[3], int32_t layers, mi::SplitScratch& sc)
+{
+    (void)m;
+    sc = mi::SplitScratch{};
+    const int budget = g_splitBudget.load();
+    if (budget <= 0 || layers <= 0) return MI_OK;
+    const int32_t d[3] = { dims[0], dims[1], layers };
+    const int64_t waves = mi::sdfGridWaves(d, 0, layers);
+    const size_t bytes = mi::splitScratchBytes(waves, 4);
+    if (buf.bytes < bytes) MI_HIP(buf.alloc(bytes));
+    mi::splitScratchBind(buf.p, waves, 4, budget, sc);
+    return MI_OK;
+}
+
+// split scratch of a projection launch over n points, placed at `base` (caller-sized with
+// projSplitBytes); no split when the budget is 0
+size_t projSplitBytes(int64_t n)
+{
+    return g_splitBudget.load() > 0 ? mi::splitScratchBytes(mi::projectionWaves(n), 8) : 0;
+}
+
+void projSplit(void* base, int64_t n, mi::SplitScratch& sc)
+{
+    sc = mi::SplitScratch{};
+    const int budget = g_splitBudget.load();
+    if (budget <= 0 || !base) return;
+    mi::splitScratchBind(base, mi::projectionWaves(n), 8, budget, sc);
+}
+
 // device-resident entry points run on the caller's stream; NULL is the HIP null (default) stream
 hipStream_t pick(mi_mesh*, void* s) { return static_cast<hipStream_t>(s); }
 
@@ -229,19 +260,29 @@ int mi_set_tree_builder(int builder)
 
 int mi_get_tree_builder(void) { return g_treeBuilder.load(); }
 
+int mi_set_split_budget(int32_t steps)
+{
+    if (steps < 0) return fail(MI_E_INVALID, "negative split budget");
+    g_splitBudget.store(steps);
+    return MI_OK;
+}
+
+int32_t mi_get_split_budget(void) { return g_splitBudget.load(); }
+
 } // extern "C"
 
 namespace {
 
 struct ArenaLayout {
-    size_t dnode, tri, dint, dleaf, dquad;
-    size_t total() const { return dnode + tri + dint + dleaf + dquad; }
+    size_t dnode, tri, dint, dleaf, dquad, cut;
+    size_t total() const { return dnode + tri + dint + dleaf + dquad + cut; }
 };
 
 ArenaLayout arenaLayout(int32_t nf)
 {
     const size_t ni = nf > 0 ? (size_t)nf - 1 : 0, nl = nf > 0 ? (size_t)nf : 0;
-    return { align256(ni * 64), align256(nl * 48), align256(ni * 32), align256(nl * 32), align256(ni * 128) };
+    return { align256(ni * 64), align256(nl * 48), align256(ni * 32), align256(nl * 32), align256(ni * 128),
+             nf > 0 ? align256(mi::kMaxCut * sizeof(mi::CutEntry)) + 256 : 0 };
 }
 
 void bindArena(mi_mesh* m, const ArenaLayout& L)
@@ -254,14 +295,25 @@ void bindArena(mi_mesh* m, const ArenaLayout& L)
     m->dev.dquad = reinterpret_cast<const float4*>(base + L.dnode + L.tri + L.dint + L.dleaf);
 }
 
-// 4-wide records of the cooperative walks, from the binary records already in the arena
+// 4-wide records of the cooperative walks and the cut of the split form, from the binary records
+// already in the arena (needs the root set)
 int finishArena(mi_mesh* m, const ArenaLayout& L)
 {
-    if (m->nf < 2) return MI_OK;
+    if (m->nf < 1) return MI_OK;
     char* base = m->arena.as<char>();
-    MI_HIP(mi::buildQuads(m->dev.dnode, m->nf - 1,
-                          reinterpret_cast<float4*>(base + L.dnode + L.tri + L.dint + L.dleaf), m->stream));
+    if (m->nf >= 2)
+        MI_HIP(mi::buildQuads(m->dev.dnode, m->nf - 1,
+                              reinterpret_cast<float4*>(base + L.dnode + L.tri + L.dint + L.dleaf), m->stream));
+    char* cutBase = base + L.dnode + L.tri + L.dint + L.dleaf + L.dquad;
+    mi::CutEntry* cut = reinterpret_cast<mi::CutEntry*>(cutBase);
+    int32_t* nCutDev = reinterpret_cast<int32_t*>(cutBase + align256(mi::kMaxCut * sizeof(mi::CutEntry)));
+    MI_HIP(mi::buildCut(m->dev.dnode, m->dev.rootRef, m->dev.rootMin, m->dev.rootMax, cut, nCutDev, m->stream));
+    int32_t nCut = 0;
+    MI_HIP(hipMemcpyAsync(&nCut, nCutDev, 4, hipMemcpyDeviceToHost, m->stream));
     MI_HIP(hipStreamSynchronize(m->stream));
+    if (nCut < 1 || nCut > mi::kMaxCut) return fail(MI_E_HIP, "AMD GPU error: bad tree cut");
+    m->dev.cut = cut;
+    m->dev.nCut = nCut;
     return MI_OK;
 }
 
@@ -304,9 +356,11 @@ int createHostBuilt(mi_mesh* m, const float* verts, const int32_t* tris)
             MI_HIP(hipMemcpy(base + L.dnode + L.tri, ft.dint.data(), ft.dint.size() * 4, hipMemcpyHostToDevice));
         MI_HIP(hipMemcpy(base + L.dnode + L.tri + L.dint, ft.dleaf.data(), ft.dleaf.size() * 4,
                          hipMemcpyHostToDevice));
+    }
+    if (m->nf > 0) {
+        setRoot(m, ft.rootBox, ft.rootRef);
         if (int rc = finishArena(m, L)) return rc;
     }
-    if (m->nf > 0) setRoot(m, ft.rootBox, ft.rootRef);
     return MI_OK;
 }
 
@@ -341,8 +395,8 @@ int createDeviceBuilt(mi_mesh* m, const float* verts, const int32_t* tris)
     m->depth = info.depth;
     m->stackD = mi::stackDepthFor(m->depth);
     if (m->stackD == 0) return fail(MI_E_LIMIT, "tree deeper than 32 levels");
-    if (int rc = finishArena(m, L)) return rc;
     setRoot(m, info.rootBox, info.rootRef);
+    if (int rc = finishArena(m, L)) return rc;
     return MI_OK;
 }
 
@@ -437,10 +491,14 @@ int mi_sdf_grid_device(mi_mesh* m, const int32_t dims[3], const float xf[12], co
     if (int rc = checkGridArgs(dims, xf)) return rc;
     if (zBegin < 0 || zEnd > dims[2] || zBegin > zEnd) return fail(MI_E_INVALID, "bad z-layer range");
     if (zEnd > zBegin && (int64_t)dims[0] * dims[1] > 0 && !d_res) return fail(MI_E_INVALID, "output is null");
+    std::lock_guard<std::mutex> lk(m->mu);
     MI_HIP(hipSetDevice(m->device));
     mi::DistOpts o{ opts->minDistSq, opts->maxDistSq, opts->nullOutsideMinMax, opts->windingNumberThreshold,
                     opts->windingNumberBeta };
-    MI_HIP(mi::launchSdfGrid(m->dev, dims, toXf(xf), o, zBegin, zEnd, d_res, m->stackD, pick(m, stream)));
+    mi::SplitScratch sc;
+    if (int rc = splitScratch(m, m->ws, dims, zEnd - zBegin, sc)) return rc;
+    MI_HIP(mi::launchSdfGrid(m->dev, dims, toXf(xf), o, zBegin, zEnd, d_res, m->stackD, pick(m, stream), nullptr,
+                             nullptr, nullptr, &sc));
     return MI_OK;
 }
 
@@ -481,14 +539,17 @@ int mi_proj_find_projections_device(mi_mesh* m, const float* d_pts, int64_t n, c
     std::lock_guard<std::mutex> lk(m->mu);
     MI_HIP(hipSetDevice(m->device));
     hipStream_t s = pick(m, stream);
-    if (int rc = ensureWorkspace(m, mi::sortWorkspaceBytes(n))) return rc;
+    const size_t sortBytes = align256(mi::sortWorkspaceBytes(n));
+    if (int rc = ensureWorkspace(m, sortBytes + projSplitBytes(n))) return rc;
+    mi::SplitScratch sc;
+    projSplit(m->ws.as<char>() + sortBytes, n, sc);
     const uint32_t* perm = nullptr;
     MI_HIP(mi::sortPointsMorton(d_pts, n, m->sortLo, m->sortHi, m->ws.p, m->ws.bytes, &perm, s));
     mi::Xf px, tx;
     if (ptXf) px = toXf(ptXf);
     if (treeXf) tx = toXf(treeXf);
     MI_HIP(mi::launchProjections(m->dev, d_pts, perm, n, ptXf ? &px : nullptr, treeXf ? &tx : nullptr, up, lo,
-                                 reinterpret_cast<mi::ProjRec*>(d_res), m->stackD, s));
+                                 reinterpret_cast<mi::ProjRec*>(d_res), m->stackD, s, nullptr, nullptr, nullptr, &sc));
     return MI_OK;
 }
 
@@ -502,14 +563,17 @@ int mi_proj_find_projections_bounded_device(mi_mesh* m, const float* d_pts, int6
     std::lock_guard<std::mutex> lk(m->mu);
     MI_HIP(hipSetDevice(m->device));
     hipStream_t s = pick(m, stream);
-    if (int rc = ensureWorkspace(m, mi::sortWorkspaceBytes(n))) return rc;
+    const size_t sortBytes = align256(mi::sortWorkspaceBytes(n));
+    if (int rc = ensureWorkspace(m, sortBytes + projSplitBytes(n))) return rc;
+    mi::SplitScratch sc;
+    projSplit(m->ws.as<char>() + sortBytes, n, sc);
     const uint32_t* perm = nullptr;
     MI_HIP(mi::sortPointsMorton(d_pts, n, m->sortLo, m->sortHi, m->ws.p, m->ws.bytes, &perm, s));
     mi::Xf px, tx;
     if (ptXf) px = toXf(ptXf);
     if (treeXf) tx = toXf(treeXf);
     MI_HIP(mi::launchProjections(m->dev, d_pts, perm, n, ptXf ? &px : nullptr, treeXf ? &tx : nullptr, up, lo,
-                                 reinterpret_cast<mi::ProjRec*>(d_res), m->stackD, s, d_upSq));
+                                 reinterpret_cast<mi::ProjRec*>(d_res), m->stackD, s, d_upSq, nullptr, nullptr, &sc));
     return MI_OK;
 }
 
@@ -589,10 +653,15 @@ static int gridHost(mi_mesh* m, const int32_t dims[3], const float xf[12], const
     if (!report(cb, user, 0.5f)) return fail(MI_E_CANCELED, "Operation was canceled");
     if (total == 0) return MI_OK;
     // maxBufferSizeAlignedByBlock (source/MRCuda/MRCudaBasic.cpp:103-107): whole z-layers
-    const size_t layers = std::min(memBudget() / sizeof(float) / layer, (size_t)dims[2]);
+    // the split form of the signed-distance walk needs about as many scratch bytes as the output
+    const size_t perVoxel = opts ? 2 * sizeof(float) : sizeof(float);
+    const size_t layers = std::min(memBudget() / perVoxel / layer, (size_t)dims[2]);
     if (layers == 0) return fail(MI_E_LIMIT, "one z-layer does not fit in the device memory budget");
-    DevBuf out;
+    DevBuf out, splitBuf;
     MI_HIP(out.alloc(layers * layer * sizeof(float)));
+    mi::SplitScratch sc;
+    if (opts)
+        if (int rc = splitScratch(m, splitBuf, dims, (int32_t)layers, sc)) return rc;
     if (!report(cb, user, 0.6f)) return fail(MI_E_CANCELED, "Operation was canceled");
     const int chunks = (int)((dims[2] + layers - 1) / layers);
     const mi::Xf x = toXf(xf);
@@ -601,7 +670,8 @@ static int gridHost(mi_mesh* m, const int32_t dims[3], const float xf[12], const
         if (opts) {
             mi::DistOpts o{ opts->minDistSq, opts->maxDistSq, opts->nullOutsideMinMax,
                             opts->windingNumberThreshold, opts->windingNumberBeta };
-            MI_HIP(mi::launchSdfGrid(m->dev, dims, x, o, z0, z1, out.as<float>(), m->stackD, m->stream));
+            MI_HIP(mi::launchSdfGrid(m->dev, dims, x, o, z0, z1, out.as<float>(), m->stackD, m->stream, nullptr,
+                                     nullptr, nullptr, &sc));
         } else {
             MI_HIP(mi::launchFwnGrid(m->dev, dims, x, beta, z0, z1, out.as<float>(), m->stackD, m->stream));
         }
@@ -642,7 +712,8 @@ static int gridByParts(mi_mesh* m, mi_grid_part_fn fn, void* fnUser, const int32
     if (!report(cb, user, 0.5f)) return fail(MI_E_CANCELED, "Operation was canceled");
     if (total == 0) return MI_OK;
     // two device buffers + two pinned host buffers, each of `chunkLayers` whole z-layers
-    const size_t chunkLayers = std::min(memBudget() / sizeof(float) / layer / 2, (size_t)dims[2]);
+    const size_t perVoxel = opts ? 2 * sizeof(float) : sizeof(float);
+    const size_t chunkLayers = std::min(memBudget() / perVoxel / layer / 2, (size_t)dims[2]);
     if (chunkLayers == 0) return fail(MI_E_LIMIT, "one z-layer does not fit in the device memory budget");
     // splitByChunks (source/MRMesh/MRChunkIterator.cpp:29-53) in units of layers
     size_t step = chunkLayers, nChunks = 1;
@@ -652,7 +723,11 @@ static int gridByParts(mi_mesh* m, mi_grid_part_fn fn, void* fnUser, const int32
         step = chunkLayers - (size_t)layerOverlap;
         nChunks = ((size_t)dims[2] - (size_t)layerOverlap + step - 1) / step;
     }
-    DevBuf dbuf[2];
+    DevBuf dbuf[2], splitBuf[2];
+    mi::SplitScratch sc[2];
+    if (opts)
+        for (int i = 0; i < 2; ++i)
+            if (int rc = splitScratch(m, splitBuf[i], dims, (int32_t)chunkLayers, sc[i])) return rc;
     struct Host {
         float* hbuf[2] = { nullptr, nullptr };
         hipStream_t side = nullptr, main = nullptr;
@@ -691,7 +766,8 @@ static int gridByParts(mi_mesh* m, mi_grid_part_fn fn, void* fnUser, const int32
         if (opts) {
             mi::DistOpts o{ opts->minDistSq, opts->maxDistSq, opts->nullOutsideMinMax,
                             opts->windingNumberThreshold, opts->windingNumberBeta };
-            MI_HIP(mi::launchSdfGrid(m->dev, dims, x, o, z0, z1, dbuf[b].as<float>(), m->stackD, st[b]));
+            MI_HIP(mi::launchSdfGrid(m->dev, dims, x, o, z0, z1, dbuf[b].as<float>(), m->stackD, st[b], nullptr,
+                                     nullptr, nullptr, &sc[b]));
         } else {
             MI_HIP(mi::launchFwnGrid(m->dev, dims, x, beta, z0, z1, dbuf[b].as<float>(), m->stackD, st[b]));
         }
@@ -745,21 +821,27 @@ int mi_fwn_calc_from_grid_with_distances_by_parts(mi_mesh* m, mi_grid_part_fn fn
 // per-point chunked pipeline shared by calcFromVector and findProjections
 template <class LaunchFn>
 static int pointsHost(mi_mesh* m, const float* pts, int64_t n, size_t outElem, void* res, LaunchFn&& launchFn,
-                      mi_progress_fn cb, void* user)
+                      mi_progress_fn cb, void* user, bool split = false)
 {
     std::lock_guard<std::mutex> lk(m->mu);
     MI_HIP(hipSetDevice(m->device));
     if (!report(cb, user, 0.5f)) return fail(MI_E_CANCELED, "Operation was canceled");
     if (n == 0) return MI_OK;
     // maxBufferSize (source/MRCuda/MRCudaBasic.cpp:94-97) with the sort workspace counted
-    const size_t perElem = 12 + outElem + 20;
+    // (+ 9 B per point for the split form's scratch of the projection walk)
+    const size_t perElem = 12 + outElem + 20 + (split ? 9 : 0);
     size_t chunk = std::min<size_t>(memBudget() / perElem, (size_t)n);
     chunk = std::min<size_t>(chunk, (size_t)INT32_MAX);
     if (chunk == 0) return fail(MI_E_LIMIT, "no point fits in the device memory budget");
-    DevBuf dpts, dres, ws;
+    DevBuf dpts, dres, ws, splitBuf;
     MI_HIP(dpts.alloc(chunk * 12));
     MI_HIP(dres.alloc(chunk * outElem));
     MI_HIP(ws.alloc(mi::sortWorkspaceBytes((int64_t)chunk)));
+    mi::SplitScratch sc;
+    if (split && projSplitBytes((int64_t)chunk)) {
+        MI_HIP(splitBuf.alloc(projSplitBytes((int64_t)chunk)));
+        projSplit(splitBuf.p, (int64_t)chunk, sc);
+    }
     if (!report(cb, user, 0.6f)) return fail(MI_E_CANCELED, "Operation was canceled");
     const size_t chunks = ((size_t)n + chunk - 1) / chunk;
     for (size_t c = 0; c < chunks; ++c) {
@@ -768,7 +850,7 @@ static int pointsHost(mi_mesh* m, const float* pts, int64_t n, size_t outElem, v
         const uint32_t* perm = nullptr;
         MI_HIP(mi::sortPointsMorton(dpts.as<float>(), (int64_t)sz, m->sortLo, m->sortHi, ws.p, ws.bytes, &perm,
                                     m->stream));
-        MI_HIP(launchFn(dpts.as<float>(), perm, (int64_t)sz, dres.p));
+        MI_HIP(launchFn(dpts.as<float>(), perm, (int64_t)sz, dres.p, &sc));
         MI_HIP(hipMemcpyAsync(static_cast<char*>(res) + off * outElem, dres.p, sz * outElem,
                               hipMemcpyDeviceToHost, m->stream));
         MI_HIP(hipStreamSynchronize(m->stream));
@@ -785,7 +867,7 @@ int mi_fwn_calc_from_vector(mi_mesh* m, const float* pts, int64_t n, float beta,
 {
     if (!m || n < 0 || (n > 0 && (!pts || !res))) return fail(MI_E_INVALID, "invalid arguments");
     return pointsHost(m, pts, n, sizeof(float), res,
-                      [&](const float* dp, const uint32_t* perm, int64_t sz, void* dr) {
+                      [&](const float* dp, const uint32_t* perm, int64_t sz, void* dr, const mi::SplitScratch*) {
                           return mi::launchFwnPoints(m->dev, dp, perm, sz, beta, skipFace, static_cast<float*>(dr),
                                                      m->stackD, m->stream);
                       },
@@ -828,17 +910,18 @@ int mi_proj_find_projections(mi_mesh* m, const float* pts, int64_t n, const floa
     if (treeXf) tx = toXf(treeXf);
     (void)kIdentityXf;
     return pointsHost(m, pts, n, sizeof(mi_projection), res,
-                      [&](const float* dp, const uint32_t* perm, int64_t sz, void* dr) {
+                      [&](const float* dp, const uint32_t* perm, int64_t sz, void* dr, const mi::SplitScratch* sc) {
                           return mi::launchProjections(m->dev, dp, perm, sz, ptXf ? &px : nullptr,
                                                        treeXf ? &tx : nullptr, up, lo,
-                                                       static_cast<mi::ProjRec*>(dr), m->stackD, m->stream);
+                                                       static_cast<mi::ProjRec*>(dr), m->stackD, m->stream,
+                                                       nullptr, nullptr, nullptr, sc);
                       },
-                      nullptr, nullptr);
+                      nullptr, nullptr, true);
 }
 
 size_t mi_proj_heap_bytes(size_t numProjections)
 {
-    return numProjections * (12 + sizeof(mi_projection) + 20);
+    return numProjections * (12 + sizeof(mi_projection) + 20 + 9);
 }
 
 } // extern "C"

@@ -45,6 +45,22 @@ __device__ __forceinline__ int logicalBlock(int nLogical)
     return (b % kXcds) * per + b / kXcds;
 }
 
+// blocks of a persistent launch: enough resident waves to fill every CU
+inline int persistentBlocks()
+{
+    static std::atomic<int> cached[64];
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    if (dev < 0 || dev >= 64) dev = 0;
+    if (!cached[dev]) {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+        cached[dev] = cus * 8;
+    }
+    return cached[dev];
+}
+
 inline int paddedGrid(int nLogical)
 {
     const int per = (nLogical + kXcds - 1) / kXcds;
@@ -510,20 +526,15 @@ __device__ __forceinline__ int rankSlots(const uint64_t b[4], const float d[4], 
 // 4-wide records: the minimum does not depend on the visit order, so it equals the per-lane
 // result bit for bit.  A step tests the four grandchildren of a node; the nearest active one (for
 // the lowest active lane) continues, the other active ones are pushed farthest first.
+// The walk of coopDistance from node `cur` (box distance `curd`), for lanes whose best so far is
+// `best`; after `budget` visited steps (0 = no limit) it stops and sets `stopped` (wave-uniform),
+// returning the best found so far -- a real candidate, so a later walk can finish from it.
 template <int D>
-__device__ __forceinline__ float coopDistance(const MeshDev& m, float px, float py, float pz, float up,
-                                              float4* stk)
+__device__ __forceinline__ float coopDistanceWalk(const MeshDev& m, float px, float py, float pz, float best,
+                                                  int cur, float curd, float4* stk, int budget, bool& stopped)
 {
-    float best = up;
-    if (m.numFaces <= 0) return best;
-    if (up > 0) {
-        const float g = greedyLeafDistSq(m, px, py, pz);
-        if (g < best) best = g;
-    }
-    int sp = 0;
-    int cur = m.rootRef;
-    float curd = boxDistSq(m.rootMin[0], m.rootMin[1], m.rootMin[2], m.rootMax[0], m.rootMax[1],
-                           m.rootMax[2], px, py, pz);
+    stopped = false;
+    int sp = 0, steps = 0;
     bool have = true;
     for (;;) {
         if (!have) {
@@ -536,6 +547,10 @@ __device__ __forceinline__ float coopDistance(const MeshDev& m, float px, float 
         have = false;
         const bool act = curd < best;
         if (!waveBallot(act)) continue;
+        if (budget && ++steps > budget) {
+            stopped = true;
+            break;
+        }
         cur = uniform(cur);
         if (cur < 0) {
             const Leaf L = loadLeaf(m.tri, ~cur);
@@ -578,6 +593,30 @@ __device__ __forceinline__ float coopDistance(const MeshDev& m, float px, float 
         sp += cnt - 1;
         have = true;
     }
+    return best;
+}
+
+// Squared distance to the nearest leaf (calcDistanceSq / findProjectionSubtree distance part,
+// source/MRCuda/MRCudaFastWindingNumber.cu:65-133) for each lane, on a wave-shared walk over the
+// 4-wide records: the minimum does not depend on the visit order, so it equals the per-lane
+// result bit for bit.  A step tests the four grandchildren of a node; the nearest active one (for
+// the lowest active lane) continues, the other active ones are pushed farthest first.
+template <int D>
+__device__ __forceinline__ float coopDistance(const MeshDev& m, float px, float py, float pz, float up,
+                                              float4* stk, int budget = 0, bool* stopped = nullptr)
+{
+    float best = up;
+    if (stopped) *stopped = false;
+    if (m.numFaces <= 0) return best;
+    if (up > 0) {
+        const float g = greedyLeafDistSq(m, px, py, pz);
+        if (g < best) best = g;
+    }
+    const float rootd = boxDistSq(m.rootMin[0], m.rootMin[1], m.rootMin[2], m.rootMax[0], m.rootMax[1],
+                                  m.rootMax[2], px, py, pz);
+    bool st = false;
+    best = coopDistanceWalk<D>(m, px, py, pz, best, m.rootRef, rootd, stk, budget, st);
+    if (stopped) *stopped = st;
     return best;
 }
 
@@ -645,32 +684,17 @@ __device__ __forceinline__ float coopWinding(const MeshDev& m, float qx, float q
 // Keys are not stored per lane: a popped entry is related to the slot popped or continued just
 // before it in the same step -- its sibling (flip the entry's bit, clear below), or a child of
 // the other child (flip the bit one level up, then the entry's own bit from a 64-lane mask).
+// The walk of coopProject from node `cur` at depth `curDepth` (box distance `curd`, per-lane key
+// prefix `curKey`), improving each lane's (res.distSq, bestKey); after `budget` visited steps (0 =
+// no limit) it stops and sets `stopped` (wave-uniform).
 template <int D>
-__device__ __forceinline__ void coopProject(const MeshDev& m, float px, float py, float pz, float up, bool valid,
-                                            float4* stk, BestHit& res)
+__device__ __forceinline__ void coopProjectWalk(const MeshDev& m, float px, float py, float pz, int cur,
+                                                int curDepth, float curd, uint32_t curKey, float4* stk,
+                                                BestHit& res, uint32_t& bestKey, int budget, bool& stopped)
 {
-    res.distSq = valid ? up : -1.f;
-    res.face = -1;
-    res.px = res.py = res.pz = 0;
-    res.a = res.b = 0;
-    uint32_t bestKey = 0;
-    if (m.numFaces <= 0) return;
-    // search radius just above the greedy leaf's distance: that leaf (and every leaf at the minimum)
-    // is still accepted, so the record is unchanged; an empty search keeps the caller's limit
-    if (valid && up > 0) {
-        const float g = greedyLeafDistSq(m, px, py, pz);
-        if (g < up) {
-            // any radius above g keeps that leaf; at least FLT_MIN so no comparison meets a denormal
-            const float above = g < FLT_MIN ? FLT_MIN : __uint_as_float(__float_as_uint(g) + 1u);
-            res.distSq = above < up ? above : up;
-        }
-    }
+    stopped = false;
     const int lane = threadIdx.x & 63;
-    int sp = 0;
-    int cur = m.rootRef, curDepth = 0;
-    float curd = boxDistSq(m.rootMin[0], m.rootMin[1], m.rootMin[2], m.rootMax[0], m.rootMax[1],
-                           m.rootMax[2], px, py, pz);
-    uint32_t curKey = 0;
+    int sp = 0, steps = 0;
     bool have = true;
     for (;;) {
         if (!have) {
@@ -694,6 +718,10 @@ __device__ __forceinline__ void coopProject(const MeshDev& m, float px, float py
         have = false;
         const bool act = curd < res.distSq || (curd == res.distSq && curKey < bestKey);
         if (!waveBallot(act)) continue;
+        if (budget && ++steps > budget) {
+            stopped = true;
+            break;
+        }
         cur = uniform(cur);
         curDepth = uniform(curDepth);
         if (cur < 0) {
@@ -719,17 +747,15 @@ __device__ __forceinline__ void coopProject(const MeshDev& m, float px, float py
             continue;
         }
         const Quad Q = loadQuad(m, cur);
+        // the two children's boxes (the node's binary record) decide this lane's first choice, the
+        // grandchildren's its second
+        const float4 n0 = m.dnode[4 * (size_t)cur], n1 = m.dnode[4 * (size_t)cur + 1];
+        const float4 n2 = m.dnode[4 * (size_t)cur + 2];
         float d[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) d[i] = slotDistSq(Q, i, px, py, pz);
-        // the two children (union of their slots; an empty slot adds nothing) decide this lane's
-        // first choice, the grandchildren its second
-        const float dA = boxDistSq(fminf(Q.mn[0][0], Q.mn[1][0]), fminf(Q.mn[0][1], Q.mn[1][1]),
-                                   fminf(Q.mn[0][2], Q.mn[1][2]), fmaxf(Q.mx[0][0], Q.mx[1][0]),
-                                   fmaxf(Q.mx[0][1], Q.mx[1][1]), fmaxf(Q.mx[0][2], Q.mx[1][2]), px, py, pz);
-        const float dB = boxDistSq(fminf(Q.mn[2][0], Q.mn[3][0]), fminf(Q.mn[2][1], Q.mn[3][1]),
-                                   fminf(Q.mn[2][2], Q.mn[3][2]), fmaxf(Q.mx[2][0], Q.mx[3][0]),
-                                   fmaxf(Q.mx[2][1], Q.mx[3][1]), fmaxf(Q.mx[2][2], Q.mx[3][2]), px, py, pz);
+        const float dA = boxDistSq(n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, px, py, pz);
+        const float dB = boxDistSq(n1.z, n1.w, n2.x, n2.y, n2.z, n2.w, px, py, pz);
         const bool nearA = dA < dB;                           // ties: right child first
         const uint32_t bit1 = 1u << (31 - curDepth), bit2 = bit1 >> 1;
         const uint32_t kA = curKey | (nearA ? 0u : bit1), kB = curKey | (nearA ? bit1 : 0u);
@@ -783,6 +809,37 @@ __device__ __forceinline__ void coopProject(const MeshDev& m, float px, float py
     }
 }
 
+template <int D>
+__device__ __forceinline__ void coopProject(const MeshDev& m, float px, float py, float pz, float up, bool valid,
+                                            float4* stk, BestHit& res, uint32_t* bestKeyOut = nullptr,
+                                            int budget = 0, bool* stopped = nullptr)
+{
+    res.distSq = valid ? up : -1.f;
+    res.face = -1;
+    res.px = res.py = res.pz = 0;
+    res.a = res.b = 0;
+    uint32_t bestKey = 0;
+    if (stopped) *stopped = false;
+    if (bestKeyOut) *bestKeyOut = 0;
+    if (m.numFaces <= 0) return;
+    // search radius just above the greedy leaf's distance: that leaf (and every leaf at the minimum)
+    // is still accepted, so the record is unchanged; an empty search keeps the caller's limit
+    if (valid && up > 0) {
+        const float g = greedyLeafDistSq(m, px, py, pz);
+        if (g < up) {
+            // any radius above g keeps that leaf; at least FLT_MIN so no comparison meets a denormal
+            const float above = g < FLT_MIN ? FLT_MIN : __uint_as_float(__float_as_uint(g) + 1u);
+            res.distSq = above < up ? above : up;
+        }
+    }
+    const float rootd = boxDistSq(m.rootMin[0], m.rootMin[1], m.rootMin[2], m.rootMax[0], m.rootMax[1],
+                                  m.rootMax[2], px, py, pz);
+    bool st = false;
+    coopProjectWalk<D>(m, px, py, pz, m.rootRef, 0, rootd, 0u, stk, res, bestKey, budget, st);
+    if (stopped) *stopped = st;
+    if (bestKeyOut) *bestKeyOut = bestKey;
+}
+
 // appends `value` of every lane with p set to list[] (one atomic per wave)
 __device__ __forceinline__ void appendWave(bool p, uint32_t value, uint32_t* list, uint32_t* count)
 {
@@ -807,7 +864,7 @@ __global__ __launch_bounds__(kBlock, 7) void kProjectionsCoop(MeshDev m, const f
                                                            const float* __restrict__ bound,
                                                            uint32_t* __restrict__ rerun,
                                                            uint32_t* __restrict__ rerunCount,
-                                                           ProjRec* __restrict__ out)
+                                                           SplitScratch sc, ProjRec* __restrict__ out)
 {
     __shared__ float4 stkMem[kWaves][3 * quadStack(D)];
     const int lb = logicalBlock(nLogical);
@@ -825,13 +882,152 @@ __global__ __launch_bounds__(kBlock, 7) void kProjectionsCoop(MeshDev m, const f
     }
     const int w = threadIdx.x >> 6;
     BestHit h;
-    coopProject<D>(m, px, py, pz, lim, valid, stkMem[w], h);
+    uint32_t key = 0;
+    bool stopped = false;
+    coopProject<D>(m, px, py, pz, lim, valid, stkMem[w], h, &key, sc.list ? sc.budget : 0, &stopped);
+    if (stopped) {
+        // hand the wave to the split pass with every lane's (distSq, key) so far
+        const int lane = threadIdx.x & 63;
+        uint32_t slot = 0;
+        if (lane == 0) slot = atomicAdd(&sc.counters[0], 1u);
+        slot = (uint32_t)uniform((int)slot);
+        if (lane == 0) sc.list[slot] = (uint32_t)(lb * kWaves + w);
+        reinterpret_cast<unsigned long long*>(sc.best)[(size_t)slot * 64 + lane] =
+            ((unsigned long long)__float_as_uint(h.distSq) << 32) | key;
+        return;
+    }
     const bool again = rerun && valid && h.face < 0 && lim < up;
     if (rerun) appendWave(again, (uint32_t)q, rerun, rerunCount);
     if (!valid || again) return;
     float4* o = reinterpret_cast<float4*>(out + q);
     o[0] = make_float4(__int_as_float(h.face), h.px, h.py, h.pz);
     o[1] = make_float4(__int_as_float(-1), h.a, h.b, h.distSq);
+}
+
+// the query of lane `lane` of projection wave `wv` (as in kProjectionsCoop)
+__device__ __forceinline__ bool projQuery(const float* __restrict__ pts, const uint32_t* __restrict__ perm, int64_t n,
+                                          const Xf& ptXf, int hasPtXf, float up, const float* __restrict__ bound,
+                                          uint32_t wv, int lane, int64_t& q, float& px, float& py, float& pz, float& lim)
+{
+    const int64_t i = (int64_t)wv * 64 + lane;
+    const bool valid = i < n;
+    q = valid ? (perm ? (int64_t)perm[i] : i) : 0;
+    px = py = pz = 0;
+    lim = up;
+    if (valid) {
+        px = pts[3 * q]; py = pts[3 * q + 1]; pz = pts[3 * q + 2];
+        if (hasPtXf) xfApply(ptXf, px, py, pz);
+        if (bound) { const float b = bound[q]; lim = b < up ? b : up; }
+    }
+    return valid;
+}
+
+// Second launch of the split projections: persistent waves take (handed-over wave, cut subtree)
+// pairs; each lane's key prefix for the subtree is its near/far choices along the subtree's path,
+// the walk starts from the lane's (distSq, key) so far, and improvements are merged with an atomic
+// minimum on distSq bits << 32 | key -- the order (distSq, key) that decides the record.
+template <int D>
+__global__ __launch_bounds__(kBlock, 7) void kProjSplit(MeshDev m, const float* __restrict__ pts,
+                                                       const uint32_t* __restrict__ perm, int64_t n, Xf ptXf,
+                                                       int hasPtXf, float up, const float* __restrict__ bound,
+                                                       SplitScratch sc)
+{
+    __shared__ float4 stkMem[kWaves][3 * quadStack(D)];
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    unsigned long long* best = reinterpret_cast<unsigned long long*>(sc.best);
+    const uint32_t nHard = sc.counters[0] < sc.cap ? sc.counters[0] : sc.cap;
+    for (;;) {
+        uint32_t t = 0;
+        if (lane == 0) t = atomicAdd(&sc.counters[1], 1u);
+        t = (uint32_t)uniform((int)t);
+        const uint32_t hIdx = t / (uint32_t)m.nCut, c = t % (uint32_t)m.nCut;
+        if (hIdx >= nHard) break;
+        int64_t q;
+        float px, py, pz, lim;
+        const bool valid = projQuery(pts, perm, n, ptXf, hasPtXf, up, bound, sc.list[hIdx], lane, q, px, py, pz, lim);
+        const unsigned long long seed = best[(size_t)hIdx * 64 + lane];
+        BestHit res;
+        res.distSq = valid ? __uint_as_float((uint32_t)(seed >> 32)) : -1.f;
+        res.face = -1;
+        res.px = res.py = res.pz = res.a = res.b = 0;
+        uint32_t bestKey = valid ? (uint32_t)seed : 0u;
+        const CutEntry E = m.cut[c];
+        // key prefix: this lane's near(0)/far(1) choice at every level of the path to the subtree
+        uint32_t key = 0;
+        int node = m.rootRef;
+        for (int l = 0; l < E.depth; ++l) {
+            const float4 n0 = m.dnode[4 * (size_t)node], n1 = m.dnode[4 * (size_t)node + 1];
+            const float4 n2 = m.dnode[4 * (size_t)node + 2], n3 = m.dnode[4 * (size_t)node + 3];
+            const float d1 = boxDistSq(n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, px, py, pz);
+            const float d2 = boxDistSq(n1.z, n1.w, n2.x, n2.y, n2.z, n2.w, px, py, pz);
+            const bool leftNear = d1 < d2;
+            const bool right = (E.path >> l) & 1u;
+            if (right == leftNear) key |= 1u << (31 - l);
+            node = right ? __float_as_int(n3.y) : __float_as_int(n3.x);
+        }
+        const float cd = boxDistSq(E.mn[0], E.mn[1], E.mn[2], E.mx[0], E.mx[1], E.mx[2], px, py, pz);
+        bool st;
+        coopProjectWalk<D>(m, px, py, pz, E.ref, E.depth, cd, key, stkMem[w], res, bestKey, 0, st);
+        if (valid && res.face >= 0) {
+            const unsigned long long packed = ((unsigned long long)__float_as_uint(res.distSq) << 32) | bestKey;
+            if (packed < seed) atomicMin(&best[(size_t)hIdx * 64 + lane], packed);
+        }
+    }
+}
+
+// Third launch: the handed-over waves' records.  The winning (distSq, key) names one leaf: walk down
+// from the root taking, at each level, the child this lane saw as near (key bit 0) or far (1).
+template <int D>
+__global__ __launch_bounds__(kBlock) void kProjSplitFinish(MeshDev m, const float* __restrict__ pts,
+                                                           const uint32_t* __restrict__ perm, int64_t n, Xf ptXf,
+                                                           int hasPtXf, float up, const float* __restrict__ bound,
+                                                           SplitScratch sc, ProjRec* __restrict__ out)
+{
+    const int lane = threadIdx.x & 63;
+    const unsigned long long* best = reinterpret_cast<const unsigned long long*>(sc.best);
+    const uint32_t nHard = sc.counters[0] < sc.cap ? sc.counters[0] : sc.cap;
+    for (;;) {
+        uint32_t t = 0;
+        if (lane == 0) t = atomicAdd(&sc.counters[2], 1u);
+        t = (uint32_t)uniform((int)t);
+        if (t >= nHard) break;
+        int64_t q;
+        float px, py, pz, lim;
+        const bool valid = projQuery(pts, perm, n, ptXf, hasPtXf, up, bound, sc.list[t], lane, q, px, py, pz, lim);
+        if (!valid) continue;
+        const unsigned long long b = best[(size_t)t * 64 + lane];
+        const float d = __uint_as_float((uint32_t)(b >> 32));
+        const uint32_t key = (uint32_t)b;
+        float4* o = reinterpret_cast<float4*>(out + q);
+        if (!(d < lim)) {
+            // nothing below the limit: the reference's empty result
+            o[0] = make_float4(__int_as_float(-1), 0.f, 0.f, 0.f);
+            o[1] = make_float4(__int_as_float(-1), 0.f, 0.f, lim);
+            continue;
+        }
+        int node = m.rootRef;
+        for (int l = 0; node >= 0 && l < 32; ++l) {
+            const float4 n0 = m.dnode[4 * (size_t)node], n1 = m.dnode[4 * (size_t)node + 1];
+            const float4 n2 = m.dnode[4 * (size_t)node + 2], n3 = m.dnode[4 * (size_t)node + 3];
+            const float d1 = boxDistSq(n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, px, py, pz);
+            const float d2 = boxDistSq(n1.z, n1.w, n2.x, n2.y, n2.z, n2.w, px, py, pz);
+            const bool leftNear = d1 < d2;
+            const bool far = (key >> (31 - l)) & 1u;
+            node = (far == leftNear) ? __float_as_int(n3.y) : __float_as_int(n3.x);
+        }
+        if (node >= 0) continue;     // a key always ends at a leaf of a tree of depth <= 32
+        const Leaf L = loadLeaf(m.tri, ~node);
+        const double pd[3] = { px, py, pz };
+        const double ad[3] = { L.a[0], L.a[1], L.a[2] };
+        const double bd[3] = { L.b[0], L.b[1], L.b[2] };
+        const double cd[3] = { L.c[0], L.c[1], L.c[2] };
+        double pr[3], ba, bb;
+        closestPointD(pd, ad, bd, cd, pr, ba, bb);
+        const float qx = (float)pr[0], qy = (float)pr[1], qz = (float)pr[2];
+        o[0] = make_float4(__int_as_float(L.face), qx, qy, qz);
+        o[1] = make_float4(__int_as_float(-1), (float)ba, (float)bb, d);
+    }
 }
 
 // shared tail of the signed-distance kernels: NaN band, winding-number sign, store
@@ -856,7 +1052,7 @@ __global__ __launch_bounds__(kBlock, 8) void kSdfGridCoop(MeshDev m, int dx, int
                                                        DistOpts o, const float* __restrict__ bound,
                                                        uint32_t* __restrict__ rerun,
                                                        uint32_t* __restrict__ rerunCount,
-                                                       float* __restrict__ out)
+                                                       SplitScratch sc, float* __restrict__ out)
 {
     __shared__ float4 stkMem[kWaves][2 * quadStack(D)];
     const int lb = logicalBlock(nLogical);
@@ -872,10 +1068,78 @@ __global__ __launch_bounds__(kBlock, 8) void kSdfGridCoop(MeshDev m, int dx, int
     const size_t idx = valid ? ((size_t)(z - zBegin) * dy + y) * dx + x : 0;
     float lim = o.maxDistSq;
     if (bound && valid) { const float b = bound[idx]; lim = b < lim ? b : lim; }
-    const float resSq = coopDistance<D>(m, px, py, pz, valid ? lim : -1.f, stkMem[w]);
+    bool stopped = false;
+    const float resSq = coopDistance<D>(m, px, py, pz, valid ? lim : -1.f, stkMem[w], sc.list ? sc.budget : 0,
+                                        &stopped);
+    if (stopped) {
+        // hand the brick to the split pass with every lane's best so far
+        uint32_t slot = 0;
+        if (lane == 0) slot = atomicAdd(&sc.counters[0], 1u);
+        slot = (uint32_t)uniform((int)slot);
+        if (lane == 0) sc.list[slot] = (uint32_t)brick;
+        sc.best[(size_t)slot * 64 + lane] = __float_as_uint(valid ? resSq : -1.f);
+        return;
+    }
     const bool again = rerun && valid && !(resSq < lim) && lim < o.maxDistSq;
     if (rerun) appendWave(again, (uint32_t)idx, rerun, rerunCount);
     sdfStore(m, px, py, pz, valid && !again, resSq, o, reinterpret_cast<uint4*>(stkMem[w]), out, idx);
+}
+
+// Second launch of the split form: persistent waves take (handed-over brick, cut subtree) pairs and
+// walk that subtree for the brick's voxels from their best so far; improvements are merged by an
+// atomic minimum on the float bits (non-negative floats order like their bit patterns).
+template <int D>
+__global__ __launch_bounds__(kBlock, 8) void kSdfSplit(MeshDev m, int dx, int dy, int zBegin, int zEnd, int nbx,
+                                                      int nby, int nbz, Xf xf, SplitScratch sc)
+{
+    __shared__ float4 stkMem[kWaves][2 * quadStack(D)];
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const uint32_t nHard = sc.counters[0] < sc.cap ? sc.counters[0] : sc.cap;
+    for (;;) {
+        uint32_t t = 0;
+        if (lane == 0) t = atomicAdd(&sc.counters[1], 1u);
+        t = (uint32_t)uniform((int)t);
+        const uint32_t h = t / (uint32_t)m.nCut, c = t % (uint32_t)m.nCut;
+        if (h >= nHard) break;
+        const int brick = (int)sc.list[h];
+        int x, y, z;
+        const bool valid = brickVoxel(brick, lane, nbx, nby, nbz, dx, dy, zBegin, zEnd, x, y, z);
+        float px = (float)x, py = (float)y, pz = (float)z;
+        xfApply(xf, px, py, pz);
+        const float best = valid ? __uint_as_float(sc.best[(size_t)h * 64 + lane]) : -1.f;
+        const CutEntry E = m.cut[c];
+        const float cd = boxDistSq(E.mn[0], E.mn[1], E.mn[2], E.mx[0], E.mx[1], E.mx[2], px, py, pz);
+        bool st;
+        const float r = coopDistanceWalk<D>(m, px, py, pz, best, E.ref, cd, stkMem[w], 0, st);
+        if (valid && r < best) atomicMin(&sc.best[(size_t)h * 64 + lane], __float_as_uint(r));
+    }
+}
+
+// Third launch: the handed-over bricks' results (sign from the winding number, NaN band, store).
+template <int D>
+__global__ __launch_bounds__(kBlock, 8) void kSdfSplitFinish(MeshDev m, int dx, int dy, int zBegin, int zEnd,
+                                                            int nbx, int nby, int nbz, Xf xf, DistOpts o,
+                                                            SplitScratch sc, float* __restrict__ out)
+{
+    __shared__ float4 stkMem[kWaves][2 * quadStack(D)];
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const uint32_t nHard = sc.counters[0] < sc.cap ? sc.counters[0] : sc.cap;
+    for (;;) {
+        uint32_t t = 0;
+        if (lane == 0) t = atomicAdd(&sc.counters[2], 1u);
+        t = (uint32_t)uniform((int)t);
+        if (t >= nHard) break;
+        const int brick = (int)sc.list[t];
+        int x, y, z;
+        const bool valid = brickVoxel(brick, lane, nbx, nby, nbz, dx, dy, zBegin, zEnd, x, y, z);
+        float px = (float)x, py = (float)y, pz = (float)z;
+        xfApply(xf, px, py, pz);
+        const size_t idx = valid ? ((size_t)(z - zBegin) * dy + y) * dx + x : 0;
+        const float resSq = __uint_as_float(sc.best[(size_t)t * 64 + lane]);
+        sdfStore(m, px, py, pz, valid, resSq, o, reinterpret_cast<uint4*>(stkMem[w]), out, idx);
+    }
 }
 
 // the listed voxels of a z-slab (ids relative to the slab), searched without any cap
@@ -1127,6 +1391,36 @@ __global__ void kBuildQuads(const float4* __restrict__ dnode, int32_t ni, float4
     q[7] = make_float4(__int_as_float(leaf[0]), __int_as_float(leaf[1]), 0.f, 0.f);
 }
 
+// Subtrees at depth kCutDepth (leaves above it stand for themselves), left to right: one thread
+// walks the top of the tree depth-first.
+__global__ void kBuildCut(const float4* __restrict__ dnode, CutEntry root, CutEntry* __restrict__ cut,
+                          int32_t* __restrict__ nCut)
+{
+    if (threadIdx.x != 0) return;
+    CutEntry stack[kCutDepth + 2];
+    int sp = 0, n = 0;
+    stack[sp++] = root;
+    while (sp > 0) {
+        const CutEntry e = stack[--sp];
+        if (e.ref < 0 || e.depth == kCutDepth) {
+            cut[n++] = e;
+            continue;
+        }
+        const float4 n0 = dnode[4 * (size_t)e.ref], n1 = dnode[4 * (size_t)e.ref + 1];
+        const float4 n2 = dnode[4 * (size_t)e.ref + 2], n3 = dnode[4 * (size_t)e.ref + 3];
+        CutEntry l = e, r = e;
+        l.ref = __float_as_int(n3.x);
+        r.ref = __float_as_int(n3.y);
+        l.depth = r.depth = e.depth + 1;
+        r.path = e.path | (1u << e.depth);
+        l.mn[0] = n0.x; l.mn[1] = n0.y; l.mn[2] = n0.z; l.mx[0] = n0.w; l.mx[1] = n1.x; l.mx[2] = n1.y;
+        r.mn[0] = n1.z; r.mn[1] = n1.w; r.mn[2] = n2.x; r.mx[0] = n2.y; r.mx[1] = n2.z; r.mx[2] = n2.w;
+        stack[sp++] = r;     // left first out
+        stack[sp++] = l;
+    }
+    *nCut = n;
+}
+
 __global__ void kFillU32(uint32_t* p, int n, uint32_t v)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1145,6 +1439,38 @@ hipError_t launchBatch(const BatchItem* items, int nItems, int64_t totalUnits, c
                            xcdSplit);
         return hipGetLastError();
     });
+}
+
+size_t splitScratchBytes(int64_t waves, int laneBytes)
+{
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+    return al((size_t)waves * 4) + al((size_t)waves * 64 * laneBytes) + 256;
+}
+
+void splitScratchBind(void* base, int64_t waves, int laneBytes, int32_t budget, SplitScratch& sc)
+{
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+    char* p = static_cast<char*>(base);
+    sc.list = reinterpret_cast<uint32_t*>(p);
+    sc.best = reinterpret_cast<uint32_t*>(p + al((size_t)waves * 4));
+    sc.counters = reinterpret_cast<uint32_t*>(p + al((size_t)waves * 4) + al((size_t)waves * 64 * laneBytes));
+    sc.cap = (uint32_t)waves;
+    sc.budget = budget;
+}
+
+int64_t projectionWaves(int64_t n)
+{
+    return n > 0 ? ((n + kBlock - 1) / kBlock) * kWaves : 0;
+}
+
+hipError_t buildCut(const float4* dnode, int32_t rootRef, const float rootMin[3], const float rootMax[3],
+                    CutEntry* cut, int32_t* nCut, hipStream_t s)
+{
+    CutEntry root{};
+    root.ref = rootRef;
+    for (int i = 0; i < 3; ++i) { root.mn[i] = rootMin[i]; root.mx[i] = rootMax[i]; }
+    hipLaunchKernelGGL(kBuildCut, dim3(1), dim3(64), 0, s, dnode, root, cut, nCut);
+    return hipGetLastError();
 }
 
 hipError_t buildQuads(const float4* dnode, int32_t numInternal, float4* dquad, hipStream_t s)
@@ -1178,9 +1504,16 @@ std::atomic<int> g_variant{ (int)Traversal::WaveCoop };
 void setTraversal(Traversal t) { g_variant.store((int)t); }
 Traversal traversal() { return (Traversal)g_variant.load(); }
 
+int64_t sdfGridWaves(const int32_t dims[3], int32_t zBegin, int32_t zEnd)
+{
+    const int64_t nbx = (dims[0] + 3) / 4, nby = (dims[1] + 3) / 4, nbz = (zEnd - zBegin + 3) / 4;
+    const int64_t bricks = nbx * nby * nbz;
+    return bricks > 0 ? ((bricks + 3) / 4) * kWaves : 0;
+}
+
 hipError_t launchSdfGrid(const MeshDev& m, const int32_t dims[3], const Xf& xf, const DistOpts& o,
                          int32_t zBegin, int32_t zEnd, float* out, int stackD, hipStream_t s,
-                         const float* bound, uint32_t* rerun, uint32_t* rerunCount)
+                         const float* bound, uint32_t* rerun, uint32_t* rerunCount, const SplitScratch* split)
 {
     const int nbx = (dims[0] + 3) / 4, nby = (dims[1] + 3) / 4, nbz = (zEnd - zBegin + 3) / 4;
     const int64_t bricks = (int64_t)nbx * nby * nbz;
@@ -1191,15 +1524,30 @@ hipError_t launchSdfGrid(const MeshDev& m, const int32_t dims[3], const Xf& xf, 
     // keeps the per-lane traversal
     const bool coop = (traversal() == Traversal::WaveCoop || bound) && !(o.minDistSq > 0);
     if (bound && !coop) return hipErrorInvalidValue;
+    SplitScratch sc;
+    if (coop && split && split->list && split->budget > 0 && !rerun && m.nCut > 0) {
+        if ((int64_t)split->cap < sdfGridWaves(dims, zBegin, zEnd)) return hipErrorInvalidValue;
+        sc = *split;
+        hipError_t e = fillU32(sc.counters, 3, 0u, s);
+        if (e != hipSuccess) return e;
+    }
     return dispatchDepth(stackD, [&](auto Dc) {
         constexpr int D = decltype(Dc)::value;
-        if (coop)
+        if (coop) {
             hipLaunchKernelGGL(kSdfGridCoop<D>, dim3(paddedGrid(nLogical)), dim3(kBlock), 0, s, m, dims[0],
                                dims[1], zBegin, zEnd, nbx, nby, nbz, nLogical, xf, o, bound, rerun, rerunCount,
-                               out);
-        else
+                               sc, out);
+            if (sc.list) {
+                const int blocks = persistentBlocks();
+                hipLaunchKernelGGL(kSdfSplit<D>, dim3(blocks), dim3(kBlock), 0, s, m, dims[0], dims[1], zBegin,
+                                   zEnd, nbx, nby, nbz, xf, sc);
+                hipLaunchKernelGGL(kSdfSplitFinish<D>, dim3(blocks), dim3(kBlock), 0, s, m, dims[0], dims[1],
+                                   zBegin, zEnd, nbx, nby, nbz, xf, o, sc, out);
+            }
+        } else {
             hipLaunchKernelGGL(kSdfGrid<D>, dim3(paddedGrid(nLogical)), dim3(kBlock), 0, s, m, dims[0],
                                dims[1], zBegin, zEnd, nbx, nby, nbz, nLogical, xf, o, out);
+        }
         return hipGetLastError();
     });
 }
@@ -1277,7 +1625,7 @@ hipError_t launchSelfIntersections(const MeshDev& m, float beta, uint8_t* out, i
 hipError_t launchProjections(const MeshDev& m, const float* pts, const uint32_t* perm, int64_t n,
                              const Xf* ptXf, const Xf* treeXf, float up, float lo, ProjRec* out,
                              int stackD, hipStream_t s, const float* bound, uint32_t* rerun,
-                             uint32_t* rerunCount)
+                             uint32_t* rerunCount, const SplitScratch* split)
 {
     if (n <= 0) return hipSuccess;
     const int nLogical = (int)((n + kBlock - 1) / kBlock);
@@ -1289,12 +1637,26 @@ hipError_t launchProjections(const MeshDev& m, const float* pts, const uint32_t*
     const bool coopOk = !(lo > 0) && !treeXf;
     const bool coop = coopOk && (traversal() == Traversal::WaveCoop || rerun);
     if (rerun && !coop) return hipErrorInvalidValue;
+    SplitScratch sc;
+    if (coop && split && split->list && split->budget > 0 && !rerun && m.nCut > 0) {
+        if ((int64_t)split->cap < projectionWaves(n)) return hipErrorInvalidValue;
+        sc = *split;
+        hipError_t e = fillU32(sc.counters, 3, 0u, s);
+        if (e != hipSuccess) return e;
+    }
     return dispatchDepth(stackD, [&](auto Dc) {
         constexpr int D = decltype(Dc)::value;
-        if (coop)
+        if (coop) {
             hipLaunchKernelGGL(kProjectionsCoop<D>, dim3(paddedGrid(nLogical)), dim3(kBlock), 0, s, m, pts,
-                               perm, n, nLogical, px, ptXf ? 1 : 0, up, bound, rerun, rerunCount, out);
-        else if (treeXf)
+                               perm, n, nLogical, px, ptXf ? 1 : 0, up, bound, rerun, rerunCount, sc, out);
+            if (sc.list) {
+                const int blocks = persistentBlocks();
+                hipLaunchKernelGGL(kProjSplit<D>, dim3(blocks), dim3(kBlock), 0, s, m, pts, perm, n, px,
+                                   ptXf ? 1 : 0, up, bound, sc);
+                hipLaunchKernelGGL(kProjSplitFinish<D>, dim3(blocks), dim3(kBlock), 0, s, m, pts, perm, n, px,
+                                   ptXf ? 1 : 0, up, bound, sc, out);
+            }
+        } else if (treeXf)
             hipLaunchKernelGGL((kProjections<true, D>), dim3(paddedGrid(nLogical)), dim3(kBlock), 0, s,
                                m, pts, perm, n, nLogical, px, ptXf ? 1 : 0, tx, up, lo, bound, out);
         else

@@ -8,12 +8,24 @@
 namespace mi {
 
 // device view of one mesh's flattened tree (layout in mi_tree.hpp)
+// A subtree root of the cut at depth kCutDepth (or a leaf above it): where the split pass starts
+// its partial walks.  path: bit l = direction taken at level l from the root (0 left, 1 right).
+struct CutEntry {
+    int32_t ref, depth;
+    uint32_t path, pad;
+    float mn[3], mx[3];
+};
+constexpr int kCutDepth = 6;
+constexpr int kMaxCut = 1 << kCutDepth;
+
 struct MeshDev {
     const float4* dnode;   // 4 per internal node
     const float4* tri;     // 3 per leaf slot
     const float4* dint;    // 2 per internal node
     const float4* dleaf;   // 2 per leaf slot
     const float4* dquad;   // 8 per internal node: its grandchildren (4-wide records of the cooperative walks)
+    const CutEntry* cut;   // nCut subtree roots covering every leaf once
+    int32_t nCut;
     float rootMin[3], rootMax[3];
     int32_t rootRef;
     int32_t numFaces;
@@ -63,9 +75,34 @@ hipError_t launchBatch(const BatchItem* items, int nItems, int64_t totalUnits, c
                        unsigned long long* tickets, int xcdSplit, int blocks, int stackD, hipStream_t s);
 
 // small fill kernel (used instead of hipMemsetAsync so streams stay graph-capture safe)
+// waves of the wave-cooperative signed-distance launch over z-layers [zBegin, zEnd)
+int64_t sdfGridWaves(const int32_t dims[3], int32_t zBegin, int32_t zEnd);
+// waves of the wave-cooperative projection launch over n points
+int64_t projectionWaves(int64_t n);
+
 // 4-wide records from the binary records: internal node j's four grandchildren, a leaf child
 // standing for itself in one slot with an empty partner slot
 hipError_t buildQuads(const float4* dnode, int32_t numInternal, float4* dquad, hipStream_t s);
+
+// the cut of the tree at depth kCutDepth into at most kMaxCut subtrees (cut[], *nCut on the device)
+hipError_t buildCut(const float4* dnode, int32_t rootRef, const float rootMin[3], const float rootMax[3],
+                    CutEntry* cut, int32_t* nCut, hipStream_t s);
+
+// Split form of the wave-cooperative walks: a wave still walking after `budget` steps stops, keeps
+// its lanes' best candidates in the scratch and is finished by a second launch that splits its walk
+// over the cut's subtrees (partial results merged by atomic minimum) and a third that stores the
+// results.  Exact: every partial result is a real candidate and the subtrees cover every leaf.
+struct SplitScratch {
+    uint32_t* list = nullptr;       // ids of the waves handed over
+    uint32_t* best = nullptr;       // 64 per handed-over wave; signed distance: float bits of the best
+                                    // squared distance; projections (as 64-bit): distSq bits << 32 | key
+    uint32_t* counters = nullptr;   // [0] waves handed over, [1] split-pass ticket, [2] finish-pass ticket
+    uint32_t cap = 0;
+    int32_t budget = 0;             // steps; 0 = never hand over
+};
+// laneBytes: 4 for the signed-distance walk, 8 for projections
+size_t splitScratchBytes(int64_t waves, int laneBytes);
+void splitScratchBind(void* base, int64_t waves, int laneBytes, int32_t budget, SplitScratch& sc);
 
 hipError_t fillU32(uint32_t* p, int n, uint32_t v, hipStream_t s);
 
@@ -77,7 +114,8 @@ int stackDepthFor(int treeDepth);
 // whose cap found nothing are appended (slab-relative ids) instead of written
 hipError_t launchSdfGrid(const MeshDev& m, const int32_t dims[3], const Xf& xf, const DistOpts& o,
                          int32_t zBegin, int32_t zEnd, float* out, int stackD, hipStream_t s,
-                         const float* bound = nullptr, uint32_t* rerun = nullptr, uint32_t* rerunCount = nullptr);
+                         const float* bound = nullptr, uint32_t* rerun = nullptr, uint32_t* rerunCount = nullptr,
+                         const SplitScratch* split = nullptr);
 // the listed slab voxels, searched without a cap
 hipError_t launchSdfVoxels(const MeshDev& m, const int32_t dims[3], const Xf& xf, const DistOpts& o,
                            int32_t zBegin, const uint32_t* list, int64_t count, float* out, int stackD,
@@ -92,7 +130,8 @@ hipError_t launchSelfIntersections(const MeshDev& m, float beta, uint8_t* out, i
 hipError_t launchProjections(const MeshDev& m, const float* pts, const uint32_t* perm, int64_t n,
                              const Xf* ptXf, const Xf* treeXf, float up, float lo, ProjRec* out,
                              int stackD, hipStream_t s, const float* bound = nullptr,
-                             uint32_t* rerun = nullptr, uint32_t* rerunCount = nullptr);
+                             uint32_t* rerun = nullptr, uint32_t* rerunCount = nullptr,
+                             const SplitScratch* split = nullptr);
 
 // spatial ordering of query points: Morton codes over box [lo, hi] then a radix sort.
 // workspace layout is private; query its size first with ws == nullptr.
