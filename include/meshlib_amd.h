@@ -105,14 +105,16 @@ int mi_get_traversal(void);
 int mi_set_tree_builder(int builder);
 int mi_get_tree_builder(void);
 
-/* Step budget of the split form of the wave-cooperative signed-distance walk, process-wide (no
- * counterpart in the reference).  A 64-voxel wave still walking after `steps` node visits hands its
- * voxels, with their best candidates so far, to a second launch that walks the tree's 64 top
- * subtrees for them in parallel; results are identical for every budget.  0 = never split.
- * Default 4096.  The device entry points of one mesh share that mesh's workspace: calls on
- * different streams must not overlap in time. */
-int     mi_set_split_budget(int32_t steps);
-int32_t mi_get_split_budget(void);
+/* Step budgets of the split form of the wave-cooperative walks, process-wide (no counterpart in the
+ * reference).  A 64-query wave still walking after `steps` node visits hands its queries, with their
+ * best candidates so far, to a second launch that walks the tree's 64 top subtrees for them in
+ * parallel; results are identical for every budget.  0 = never split.  Defaults: 3072 for the
+ * signed-distance walk, 64 for projections.  The device entry points of one mesh share that mesh's
+ * workspace: calls on different streams must not overlap in time. */
+#define MI_SPLIT_SDF 0
+#define MI_SPLIT_PROJECTIONS 1
+int     mi_set_split_budget(int32_t walk, int32_t steps);
+int32_t mi_get_split_budget(int32_t walk);
 
 /* ---------------------------------------------------------------- mesh data */
 

@@ -96,8 +96,8 @@ def lib():
         "mi_get_traversal": (C.c_int, []),
         "mi_set_tree_builder": (C.c_int, [C.c_int]),
         "mi_get_tree_builder": (C.c_int, []),
-        "mi_set_split_budget": (C.c_int, [I32]),
-        "mi_get_split_budget": (I32, []),
+        "mi_set_split_budget": (C.c_int, [I32, I32]),
+        "mi_get_split_budget": (I32, [I32]),
         "mi_mesh_create": (C.c_int, [P, I32, P, I32, C.POINTER(P)]),
         "mi_mesh_destroy": (None, [P]),
         "mi_mesh_num_nodes": (C.c_int, [P]),
@@ -210,14 +210,18 @@ def set_tree_builder(builder: str):
     _check(lib().mi_set_tree_builder(TREE_BUILDERS[builder]))
 
 
-def set_split_budget(steps: int):
-    """Steps after which a signed-distance wave hands its walk to the split pass (0 = never);
-    results do not depend on it."""
-    _check(lib().mi_set_split_budget(int(steps)))
+SPLIT_WALKS = {"sdf": 0, "projections": 1}
 
 
-def get_split_budget() -> int:
-    return int(lib().mi_get_split_budget())
+def set_split_budget(steps: int, walk: str = "both"):
+    """Steps after which a wave hands its walk to the split pass (0 = never), for the signed-distance
+    walk ('sdf'), the projection walk ('projections') or both; results do not depend on it."""
+    for w in (SPLIT_WALKS if walk == "both" else [walk]):
+        _check(lib().mi_set_split_budget(SPLIT_WALKS[w], int(steps)))
+
+
+def get_split_budget(walk: str = "sdf") -> int:
+    return int(lib().mi_get_split_budget(SPLIT_WALKS[walk]))
 
 
 def get_tree_builder() -> str:

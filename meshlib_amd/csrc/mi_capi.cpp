@@ -24,7 +24,8 @@ namespace {
 thread_local std::string g_err;
 std::atomic<size_t> g_memLimit{ 0 };
 std::atomic<int> g_treeBuilder{ MI_TREE_BUILDER_DEVICE };
-std::atomic<int> g_splitBudget{ 4096 };
+std::atomic<int> g_splitBudget{ 3072 };       // signed-distance walks (dense, coherent bricks)
+std::atomic<int> g_splitBudgetProj{ 64 };     // projection walks (sparser, less coherent waves)
 
 int fail(int code, const std::string& msg)
 {
@@ -198,13 +199,13 @@ int splitScratch(mi_mesh* m, DevBuf& buf, const int32_t dims[3], int32_t layers,
 // projSplitBytes); no split when the budget is 0
 size_t projSplitBytes(int64_t n)
 {
-    return g_splitBudget.load() > 0 ? mi::splitScratchBytes(mi::projectionWaves(n), 8) : 0;
+    return g_splitBudgetProj.load() > 0 ? mi::splitScratchBytes(mi::projectionWaves(n), 8) : 0;
 }
 
 void projSplit(void* base, int64_t n, mi::SplitScratch& sc)
 {
     sc = mi::SplitScratch{};
-    const int budget = g_splitBudget.load();
+    const int budget = g_splitBudgetProj.load();
     if (budget <= 0 || !base) return;
     mi::splitScratchBind(base, mi::projectionWaves(n), 8, budget, sc);
 }
@@ -260,14 +261,19 @@ int mi_set_tree_builder(int builder)
 
 int mi_get_tree_builder(void) { return g_treeBuilder.load(); }
 
-int mi_set_split_budget(int32_t steps)
+int mi_set_split_budget(int32_t walk, int32_t steps)
 {
     if (steps < 0) return fail(MI_E_INVALID, "negative split budget");
-    g_splitBudget.store(steps);
+    if (walk == MI_SPLIT_SDF) g_splitBudget.store(steps);
+    else if (walk == MI_SPLIT_PROJECTIONS) g_splitBudgetProj.store(steps);
+    else return fail(MI_E_INVALID, "unknown walk kind");
     return MI_OK;
 }
 
-int32_t mi_get_split_budget(void) { return g_splitBudget.load(); }
+int32_t mi_get_split_budget(int32_t walk)
+{
+    return walk == MI_SPLIT_PROJECTIONS ? g_splitBudgetProj.load() : walk == MI_SPLIT_SDF ? g_splitBudget.load() : -1;
+}
 
 } // extern "C"
 

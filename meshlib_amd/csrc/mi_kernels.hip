@@ -840,6 +840,27 @@ __device__ __forceinline__ void coopProject(const MeshDev& m, float px, float py
     if (bestKeyOut) *bestKeyOut = bestKey;
 }
 
+// The cut's subtrees ordered by box distance to the first valid lane's query (ties by index), for
+// handed-over wave `slot`: lane c ranks subtree c, the lane of rank r writes order[r].
+__device__ __forceinline__ void rankCut(const MeshDev& m, bool valid, float px, float py, float pz, uint8_t* order)
+{
+    const int lane = threadIdx.x & 63;
+    const uint64_t vm = waveBallot(valid);
+    const int lead = vm ? __builtin_ctzll(vm) : 0;
+    const float lx = laneValue(px, lead), ly = laneValue(py, lead), lz = laneValue(pz, lead);
+    float d = INFINITY;
+    if (lane < m.nCut) {
+        const CutEntry& E = m.cut[lane];
+        d = boxDistSq(E.mn[0], E.mn[1], E.mn[2], E.mx[0], E.mx[1], E.mx[2], lx, ly, lz);
+    }
+    int rank = 0;
+    for (int j = 0; j < m.nCut; ++j) {
+        const float dj = laneValue(d, j);
+        rank += (dj < d || (dj == d && j < lane)) ? 1 : 0;
+    }
+    if (lane < m.nCut) order[rank] = (uint8_t)lane;
+}
+
 // appends `value` of every lane with p set to list[] (one atomic per wave)
 __device__ __forceinline__ void appendWave(bool p, uint32_t value, uint32_t* list, uint32_t* count)
 {
@@ -894,6 +915,7 @@ __global__ __launch_bounds__(kBlock, 7) void kProjectionsCoop(MeshDev m, const f
         if (lane == 0) sc.list[slot] = (uint32_t)(lb * kWaves + w);
         reinterpret_cast<unsigned long long*>(sc.best)[(size_t)slot * 64 + lane] =
             ((unsigned long long)__float_as_uint(h.distSq) << 32) | key;
+        rankCut(m, valid, px, py, pz, sc.order + (size_t)slot * 64);
         return;
     }
     const bool again = rerun && valid && h.face < 0 && lim < up;
@@ -941,8 +963,9 @@ __global__ __launch_bounds__(kBlock, 7) void kProjSplit(MeshDev m, const float* 
         uint32_t t = 0;
         if (lane == 0) t = atomicAdd(&sc.counters[1], 1u);
         t = (uint32_t)uniform((int)t);
-        const uint32_t hIdx = t / (uint32_t)m.nCut, c = t % (uint32_t)m.nCut;
+        const uint32_t hIdx = t / (uint32_t)m.nCut, r = t % (uint32_t)m.nCut;
         if (hIdx >= nHard) break;
+        const uint32_t c = sc.order[(size_t)hIdx * 64 + r];
         int64_t q;
         float px, py, pz, lim;
         const bool valid = projQuery(pts, perm, n, ptXf, hasPtXf, up, bound, sc.list[hIdx], lane, q, px, py, pz, lim);
@@ -953,6 +976,8 @@ __global__ __launch_bounds__(kBlock, 7) void kProjSplit(MeshDev m, const float* 
         res.px = res.py = res.pz = res.a = res.b = 0;
         uint32_t bestKey = valid ? (uint32_t)seed : 0u;
         const CutEntry E = m.cut[c];
+        const float cd = boxDistSq(E.mn[0], E.mn[1], E.mn[2], E.mx[0], E.mx[1], E.mx[2], px, py, pz);
+        if (!waveBallot(cd <= res.distSq)) continue;     // no lane can improve inside this subtree
         // key prefix: this lane's near(0)/far(1) choice at every level of the path to the subtree
         uint32_t key = 0;
         int node = m.rootRef;
@@ -966,7 +991,6 @@ __global__ __launch_bounds__(kBlock, 7) void kProjSplit(MeshDev m, const float* 
             if (right == leftNear) key |= 1u << (31 - l);
             node = right ? __float_as_int(n3.y) : __float_as_int(n3.x);
         }
-        const float cd = boxDistSq(E.mn[0], E.mn[1], E.mn[2], E.mx[0], E.mx[1], E.mx[2], px, py, pz);
         bool st;
         coopProjectWalk<D>(m, px, py, pz, E.ref, E.depth, cd, key, stkMem[w], res, bestKey, 0, st);
         if (valid && res.face >= 0) {
@@ -1078,6 +1102,7 @@ __global__ __launch_bounds__(kBlock, 8) void kSdfGridCoop(MeshDev m, int dx, int
         slot = (uint32_t)uniform((int)slot);
         if (lane == 0) sc.list[slot] = (uint32_t)brick;
         sc.best[(size_t)slot * 64 + lane] = __float_as_uint(valid ? resSq : -1.f);
+        rankCut(m, valid, px, py, pz, sc.order + (size_t)slot * 64);
         return;
     }
     const bool again = rerun && valid && !(resSq < lim) && lim < o.maxDistSq;
@@ -1100,8 +1125,11 @@ __global__ __launch_bounds__(kBlock, 8) void kSdfSplit(MeshDev m, int dx, int dy
         uint32_t t = 0;
         if (lane == 0) t = atomicAdd(&sc.counters[1], 1u);
         t = (uint32_t)uniform((int)t);
-        const uint32_t h = t / (uint32_t)m.nCut, c = t % (uint32_t)m.nCut;
+        // a handed-over wave's subtrees in consecutive tickets (its queries and their tree region stay
+        // in cache), nearest first
+        const uint32_t h = t / (uint32_t)m.nCut, round = t % (uint32_t)m.nCut;
         if (h >= nHard) break;
+        const uint32_t c = sc.order[(size_t)h * 64 + round];
         const int brick = (int)sc.list[h];
         int x, y, z;
         const bool valid = brickVoxel(brick, lane, nbx, nby, nbz, dx, dy, zBegin, zEnd, x, y, z);
@@ -1444,7 +1472,7 @@ hipError_t launchBatch(const BatchItem* items, int nItems, int64_t totalUnits, c
 size_t splitScratchBytes(int64_t waves, int laneBytes)
 {
     auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
-    return al((size_t)waves * 4) + al((size_t)waves * 64 * laneBytes) + 256;
+    return al((size_t)waves * 4) + al((size_t)waves * 64 * laneBytes) + al((size_t)waves * 64) + 256;
 }
 
 void splitScratchBind(void* base, int64_t waves, int laneBytes, int32_t budget, SplitScratch& sc)
@@ -1452,8 +1480,12 @@ void splitScratchBind(void* base, int64_t waves, int laneBytes, int32_t budget, 
     auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
     char* p = static_cast<char*>(base);
     sc.list = reinterpret_cast<uint32_t*>(p);
-    sc.best = reinterpret_cast<uint32_t*>(p + al((size_t)waves * 4));
-    sc.counters = reinterpret_cast<uint32_t*>(p + al((size_t)waves * 4) + al((size_t)waves * 64 * laneBytes));
+    p += al((size_t)waves * 4);
+    sc.best = reinterpret_cast<uint32_t*>(p);
+    p += al((size_t)waves * 64 * laneBytes);
+    sc.order = reinterpret_cast<uint8_t*>(p);
+    p += al((size_t)waves * 64);
+    sc.counters = reinterpret_cast<uint32_t*>(p);
     sc.cap = (uint32_t)waves;
     sc.budget = budget;
 }

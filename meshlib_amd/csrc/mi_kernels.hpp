@@ -97,6 +97,7 @@ struct SplitScratch {
     uint32_t* best = nullptr;       // 64 per handed-over wave; signed distance: float bits of the best
                                     // squared distance; projections (as 64-bit): distSq bits << 32 | key
     uint32_t* counters = nullptr;   // [0] waves handed over, [1] split-pass ticket, [2] finish-pass ticket
+    uint8_t* order = nullptr;       // 64 per handed-over wave: cut subtrees nearest first
     uint32_t cap = 0;
     int32_t budget = 0;             // steps; 0 = never hand over
 };

@@ -20,9 +20,10 @@ def say(*a):
 
 @pytest.fixture
 def restore_budget():
-    old = M.get_split_budget()
+    old = (M.get_split_budget("sdf"), M.get_split_budget("projections"))
     yield
-    M.set_split_budget(old)
+    M.set_split_budget(old[0], "sdf")
+    M.set_split_budget(old[1], "projections")
 
 
 @pytest.mark.parametrize("level,bumps,grid", [(5, 8, 64), (8, 32, 96)])
