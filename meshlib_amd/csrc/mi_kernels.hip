@@ -36,6 +36,27 @@ constexpr int quadStack(int D) { return 3 * (D / 2 + 1); }
 constexpr int kXcds = 8;
 constexpr float kInv4Pi = 1.0f / (4 * 3.14159265358979f);   // INV_4PI, MRDipole.cpp:125
 
+// Mesh records are read-only while any kernel runs: reading them through the constant address
+// space lets the compiler use scalar loads for wave-uniform refs even in kernels that also write
+// (the split passes' atomics), instead of 64 identical vector loads.
+typedef __attribute__((address_space(4))) const float cfloat;
+typedef __attribute__((address_space(4))) const int32_t cint;
+__device__ __forceinline__ float4 cload(const float4* p, size_t i)
+{
+    const cfloat* f = (const cfloat*)p + 4 * i;
+    return make_float4(f[0], f[1], f[2], f[3]);
+}
+
+__device__ __forceinline__ CutEntry loadCut(const MeshDev& m, uint32_t c)
+{
+    const cint* e = (const cint*)(m.cut + c);
+    const cfloat* f = (const cfloat*)(m.cut + c);
+    CutEntry E;
+    E.ref = e[0]; E.depth = e[1]; E.path = (uint32_t)e[2]; E.pad = 0;
+    for (int i = 0; i < 3; ++i) { E.mn[i] = f[4 + i]; E.mx[i] = f[7 + i]; }
+    return E;
+}
+
 __device__ __forceinline__ int logicalBlock(int nLogical)
 {
     // blocks are dealt to XCDs round-robin; give XCD x the contiguous range
@@ -108,7 +129,7 @@ struct Leaf { float a[3], b[3], c[3]; int32_t face; };
 
 __device__ __forceinline__ Leaf loadLeaf(const float4* __restrict__ tri, int k)
 {
-    const float4 t0 = tri[3 * k], t1 = tri[3 * k + 1], t2 = tri[3 * k + 2];
+    const float4 t0 = cload(tri, 3 * (size_t)k), t1 = cload(tri, 3 * (size_t)k + 1), t2 = cload(tri, 3 * (size_t)k + 2);
     Leaf L;
     L.a[0] = t0.x; L.a[1] = t0.y; L.a[2] = t0.z;
     L.b[0] = t0.w; L.b[1] = t1.x; L.b[2] = t1.y;
@@ -245,8 +266,8 @@ __device__ __forceinline__ void distanceDfs(const MeshDev& m, float px, float py
             }
             continue;
         }
-        const float4 n0 = m.dnode[4 * cur], n1 = m.dnode[4 * cur + 1];
-        const float4 n2 = m.dnode[4 * cur + 2], n3 = m.dnode[4 * cur + 3];
+        const float4 n0 = cload(m.dnode, 4 * cur), n1 = cload(m.dnode, 4 * cur + 1);
+        const float4 n2 = cload(m.dnode, 4 * cur + 2), n3 = cload(m.dnode, 4 * cur + 3);
         float lmn[3] = { n0.x, n0.y, n0.z }, lmx[3] = { n0.w, n1.x, n1.y };
         float rmn2[3] = { n1.z, n1.w, n2.x }, rmx2[3] = { n2.y, n2.z, n2.w };
         if (XF) { xfBox(treeXf, lmn, lmx); xfBox(treeXf, rmn2, rmx2); }
@@ -277,8 +298,8 @@ __device__ __forceinline__ float windingNumber(const MeshDev& m, float qx, float
     int32_t cur = m.rootRef;
     for (;;) {
         float4 d0, d1;
-        if (cur >= 0) { d0 = m.dint[2 * cur]; d1 = m.dint[2 * cur + 1]; }
-        else { d0 = m.dleaf[2 * (~cur)]; d1 = m.dleaf[2 * (~cur) + 1]; }
+        if (cur >= 0) { d0 = cload(m.dint, 2 * cur); d1 = cload(m.dint, 2 * cur + 1); }
+        else { d0 = cload(m.dleaf, 2 * (~cur)); d1 = cload(m.dleaf, 2 * (~cur) + 1); }
         // Dipole::addIfGoodApprox (source/MRMesh/MRDipole.h:18-27)
         const float dpx = d0.x - qx, dpy = d0.y - qy, dpz = d0.z - qz;
         const float dd = dpx * dpx + dpy * dpy + dpz * dpz;
@@ -286,7 +307,7 @@ __device__ __forceinline__ float windingNumber(const MeshDev& m, float qx, float
             const float dist = sqrtf(dd);
             if (dist > 0) res += (dpx * d1.x + dpy * d1.y + dpz * d1.z) / (dist * dd);
         } else if (cur >= 0) {
-            const float4 n3 = m.dnode[4 * cur + 3];
+            const float4 n3 = cload(m.dnode, 4 * cur + 3);
             stk[sp * kBlock].x = __float_as_int(n3.y);   // right child later
             ++sp;
             cur = __float_as_int(n3.x);                   // left child next
@@ -452,8 +473,8 @@ __device__ __forceinline__ float greedyLeafDistSq(const MeshDev& m, float px, fl
     const double pd[3] = { px, py, pz };
     int cur = m.rootRef;
     while (cur >= 0) {
-        const float4 n0 = m.dnode[4 * cur], n1 = m.dnode[4 * cur + 1];
-        const float4 n2 = m.dnode[4 * cur + 2], n3 = m.dnode[4 * cur + 3];
+        const float4 n0 = cload(m.dnode, 4 * cur), n1 = cload(m.dnode, 4 * cur + 1);
+        const float4 n2 = cload(m.dnode, 4 * cur + 2), n3 = cload(m.dnode, 4 * cur + 3);
         const float d1 = boxDistSq(n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, px, py, pz);
         const float d2 = boxDistSq(n1.z, n1.w, n2.x, n2.y, n2.z, n2.w, px, py, pz);
         cur = d1 <= d2 ? __float_as_int(n3.x) : __float_as_int(n3.y);
@@ -480,8 +501,10 @@ struct Quad {
 
 __device__ __forceinline__ Quad loadQuad(const MeshDev& m, int j)
 {
-    const float4* q = m.dquad + 8 * (size_t)j;
-    const float4 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3], q4 = q[4], q5 = q[5], q6 = q[6], q7 = q[7];
+    const size_t b = 8 * (size_t)j;
+    const float4 q0 = cload(m.dquad, b), q1 = cload(m.dquad, b + 1), q2 = cload(m.dquad, b + 2);
+    const float4 q3 = cload(m.dquad, b + 3), q4 = cload(m.dquad, b + 4), q5 = cload(m.dquad, b + 5);
+    const float4 q6 = cload(m.dquad, b + 6), q7 = cload(m.dquad, b + 7);
     Quad Q;
     Q.mn[0][0] = q0.x; Q.mn[0][1] = q0.y; Q.mn[0][2] = q0.z; Q.mx[0][0] = q0.w; Q.mx[0][1] = q1.x; Q.mx[0][2] = q1.y;
     Q.mn[1][0] = q1.z; Q.mn[1][1] = q1.w; Q.mn[1][2] = q2.x; Q.mx[1][0] = q2.y; Q.mx[1][1] = q2.z; Q.mx[1][2] = q2.w;
@@ -638,8 +661,8 @@ __device__ __forceinline__ float coopWinding(const MeshDev& m, float qx, float q
         const bool mine = (curMask >> lane) & 1;
         cur = uniform(cur);
         float4 d0, d1;
-        if (cur >= 0) { d0 = m.dint[2 * cur]; d1 = m.dint[2 * cur + 1]; }
-        else { d0 = m.dleaf[2 * (~cur)]; d1 = m.dleaf[2 * (~cur) + 1]; }
+        if (cur >= 0) { d0 = cload(m.dint, 2 * cur); d1 = cload(m.dint, 2 * cur + 1); }
+        else { d0 = cload(m.dleaf, 2 * (~cur)); d1 = cload(m.dleaf, 2 * (~cur) + 1); }
         const float dpx = d0.x - qx, dpy = d0.y - qy, dpz = d0.z - qz;
         const float dd = dpx * dpx + dpy * dpy + dpz * dpz;
         const bool approx = !(dd <= betaSq * d0.w);
@@ -650,7 +673,7 @@ __device__ __forceinline__ float coopWinding(const MeshDev& m, float qx, float q
         const uint64_t desc = waveBallot(mine && !approx);
         if (desc) {
             if (cur >= 0) {
-                const float4 n3 = m.dnode[4 * cur + 3];
+                const float4 n3 = cload(m.dnode, 4 * cur + 3);
                 stk[sp] = make_uint4(__float_as_uint(n3.y), (uint32_t)desc, (uint32_t)(desc >> 32), 0u);
                 ++sp;
                 cur = __float_as_int(n3.x);
@@ -749,8 +772,8 @@ __device__ __forceinline__ void coopProjectWalk(const MeshDev& m, float px, floa
         const Quad Q = loadQuad(m, cur);
         // the two children's boxes (the node's binary record) decide this lane's first choice, the
         // grandchildren's its second
-        const float4 n0 = m.dnode[4 * (size_t)cur], n1 = m.dnode[4 * (size_t)cur + 1];
-        const float4 n2 = m.dnode[4 * (size_t)cur + 2];
+        const float4 n0 = cload(m.dnode, 4 * (size_t)cur), n1 = cload(m.dnode, 4 * (size_t)cur + 1);
+        const float4 n2 = cload(m.dnode, 4 * (size_t)cur + 2);
         float d[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) d[i] = slotDistSq(Q, i, px, py, pz);
@@ -975,15 +998,15 @@ __global__ __launch_bounds__(kBlock, 7) void kProjSplit(MeshDev m, const float* 
         res.face = -1;
         res.px = res.py = res.pz = res.a = res.b = 0;
         uint32_t bestKey = valid ? (uint32_t)seed : 0u;
-        const CutEntry E = m.cut[c];
+        const CutEntry E = loadCut(m, c);
         const float cd = boxDistSq(E.mn[0], E.mn[1], E.mn[2], E.mx[0], E.mx[1], E.mx[2], px, py, pz);
         if (!waveBallot(cd <= res.distSq)) continue;     // no lane can improve inside this subtree
         // key prefix: this lane's near(0)/far(1) choice at every level of the path to the subtree
         uint32_t key = 0;
         int node = m.rootRef;
         for (int l = 0; l < E.depth; ++l) {
-            const float4 n0 = m.dnode[4 * (size_t)node], n1 = m.dnode[4 * (size_t)node + 1];
-            const float4 n2 = m.dnode[4 * (size_t)node + 2], n3 = m.dnode[4 * (size_t)node + 3];
+            const float4 n0 = cload(m.dnode, 4 * (size_t)node), n1 = cload(m.dnode, 4 * (size_t)node + 1);
+            const float4 n2 = cload(m.dnode, 4 * (size_t)node + 2), n3 = cload(m.dnode, 4 * (size_t)node + 3);
             const float d1 = boxDistSq(n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, px, py, pz);
             const float d2 = boxDistSq(n1.z, n1.w, n2.x, n2.y, n2.z, n2.w, px, py, pz);
             const bool leftNear = d1 < d2;
@@ -1032,8 +1055,8 @@ __global__ __launch_bounds__(kBlock) void kProjSplitFinish(MeshDev m, const floa
         }
         int node = m.rootRef;
         for (int l = 0; node >= 0 && l < 32; ++l) {
-            const float4 n0 = m.dnode[4 * (size_t)node], n1 = m.dnode[4 * (size_t)node + 1];
-            const float4 n2 = m.dnode[4 * (size_t)node + 2], n3 = m.dnode[4 * (size_t)node + 3];
+            const float4 n0 = cload(m.dnode, 4 * (size_t)node), n1 = cload(m.dnode, 4 * (size_t)node + 1);
+            const float4 n2 = cload(m.dnode, 4 * (size_t)node + 2), n3 = cload(m.dnode, 4 * (size_t)node + 3);
             const float d1 = boxDistSq(n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, px, py, pz);
             const float d2 = boxDistSq(n1.z, n1.w, n2.x, n2.y, n2.z, n2.w, px, py, pz);
             const bool leftNear = d1 < d2;
@@ -1136,7 +1159,7 @@ __global__ __launch_bounds__(kBlock, 8) void kSdfSplit(MeshDev m, int dx, int dy
         float px = (float)x, py = (float)y, pz = (float)z;
         xfApply(xf, px, py, pz);
         const float best = valid ? __uint_as_float(sc.best[(size_t)h * 64 + lane]) : -1.f;
-        const CutEntry E = m.cut[c];
+        const CutEntry E = loadCut(m, c);
         const float cd = boxDistSq(E.mn[0], E.mn[1], E.mn[2], E.mx[0], E.mx[1], E.mx[2], px, py, pz);
         bool st;
         const float r = coopDistanceWalk<D>(m, px, py, pz, best, E.ref, cd, stkMem[w], 0, st);
