@@ -902,7 +902,7 @@ __device__ __forceinline__ void appendWave(bool p, uint32_t value, uint32_t* lis
 // appended to the list, to be searched again without the cap (any cap above the true distance
 // gives the reference's record, so the combination is exact).
 template <int D>
-__global__ __launch_bounds__(kBlock, 7) void kProjectionsCoop(MeshDev m, const float* __restrict__ pts,
+__global__ __launch_bounds__(kBlock) void kProjectionsCoop(MeshDev m, const float* __restrict__ pts,
                                                            const uint32_t* __restrict__ perm, int64_t n,
                                                            int nLogical, Xf ptXf, int hasPtXf, float up,
                                                            const float* __restrict__ bound,
@@ -972,7 +972,7 @@ __device__ __forceinline__ bool projQuery(const float* __restrict__ pts, const u
 // the walk starts from the lane's (distSq, key) so far, and improvements are merged with an atomic
 // minimum on distSq bits << 32 | key -- the order (distSq, key) that decides the record.
 template <int D>
-__global__ __launch_bounds__(kBlock, 7) void kProjSplit(MeshDev m, const float* __restrict__ pts,
+__global__ __launch_bounds__(kBlock) void kProjSplit(MeshDev m, const float* __restrict__ pts,
                                                        const uint32_t* __restrict__ perm, int64_t n, Xf ptXf,
                                                        int hasPtXf, float up, const float* __restrict__ bound,
                                                        SplitScratch sc)
@@ -1094,7 +1094,7 @@ __device__ __forceinline__ void sdfStore(const MeshDev& m, float px, float py, f
 }
 
 template <int D>
-__global__ __launch_bounds__(kBlock, 8) void kSdfGridCoop(MeshDev m, int dx, int dy, int zBegin, int zEnd,
+__global__ __launch_bounds__(kBlock) void kSdfGridCoop(MeshDev m, int dx, int dy, int zBegin, int zEnd,
                                                        int nbx, int nby, int nbz, int nLogical, Xf xf,
                                                        DistOpts o, const float* __restrict__ bound,
                                                        uint32_t* __restrict__ rerun,
@@ -1137,7 +1137,7 @@ __global__ __launch_bounds__(kBlock, 8) void kSdfGridCoop(MeshDev m, int dx, int
 // walk that subtree for the brick's voxels from their best so far; improvements are merged by an
 // atomic minimum on the float bits (non-negative floats order like their bit patterns).
 template <int D>
-__global__ __launch_bounds__(kBlock, 8) void kSdfSplit(MeshDev m, int dx, int dy, int zBegin, int zEnd, int nbx,
+__global__ __launch_bounds__(kBlock) void kSdfSplit(MeshDev m, int dx, int dy, int zBegin, int zEnd, int nbx,
                                                       int nby, int nbz, Xf xf, SplitScratch sc)
 {
     __shared__ float4 stkMem[kWaves][2 * quadStack(D)];
@@ -1169,7 +1169,7 @@ __global__ __launch_bounds__(kBlock, 8) void kSdfSplit(MeshDev m, int dx, int dy
 
 // Third launch: the handed-over bricks' results (sign from the winding number, NaN band, store).
 template <int D>
-__global__ __launch_bounds__(kBlock, 8) void kSdfSplitFinish(MeshDev m, int dx, int dy, int zBegin, int zEnd,
+__global__ __launch_bounds__(kBlock) void kSdfSplitFinish(MeshDev m, int dx, int dy, int zBegin, int zEnd,
                                                             int nbx, int nby, int nbz, Xf xf, DistOpts o,
                                                             SplitScratch sc, float* __restrict__ out)
 {
@@ -1195,7 +1195,7 @@ __global__ __launch_bounds__(kBlock, 8) void kSdfSplitFinish(MeshDev m, int dx, 
 
 // the listed voxels of a z-slab (ids relative to the slab), searched without any cap
 template <int D>
-__global__ __launch_bounds__(kBlock, 8) void kSdfVoxelsCoop(MeshDev m, int dx, int dy, int zBegin,
+__global__ __launch_bounds__(kBlock) void kSdfVoxelsCoop(MeshDev m, int dx, int dy, int zBegin,
                                                          const uint32_t* __restrict__ list, int64_t count,
                                                          int nLogical, Xf xf, DistOpts o,
                                                          float* __restrict__ out)
@@ -1285,7 +1285,7 @@ __global__ __launch_bounds__(kBlock) void kSelfIntersectionsCoop(MeshDev m, int 
 // wave whose part is drained moves on to the next part: spatially adjacent units share an L2,
 // and the tail is balanced.  Every wave exits once it has seen every part drained.
 template <int D>
-__global__ __launch_bounds__(kBlock, 6) void kBatch(const BatchItem* __restrict__ items, int nItems, int64_t total,
+__global__ __launch_bounds__(kBlock) void kBatch(const BatchItem* __restrict__ items, int nItems, int64_t total,
                                                  DistOpts o, float up, unsigned long long* __restrict__ tickets,
                                                  int xcdSplit)
 {
