@@ -31,13 +31,14 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-# HBM traffic of one kSdfGridCoop launch on the default workload, from two rocprofv3 --pmc passes
-# over `bench.py --steps 1 --warmup 0 --no-cpu --no-learned --no-plain --copy-calibration`
-# (profiles/sdf_grid_coop_k8_pmc.md): FETCH_SIZE and WRITE_SIZE in KiB per dispatch, each scaled
-# by bytes / counter reading of the 1 GiB device copy in the same run.
-PMC_SDF_KIB = {"fetch": 20453141.9, "write": 151558.2}
-PMC_COPY_KIB = {"fetch": 524308.4, "write": 1048576.0}
-PMC_SDF_L2_HIT = 600924019.0 / (600924019.0 + 182379804.0)    # TCC_HIT_sum / (HIT + MISS)
+# HBM traffic of one signed-distance call on the default workload (its three launches: kSdfGridCoop,
+# kSdfSplit, kSdfSplitFinish), from rocprofv3 --pmc passes over
+# `bench.py --steps 1 --warmup 0 --no-cpu --no-learned --no-plain --copy-calibration`
+# (profiles/sdf_grid_coop_k8_pmc.md): FETCH_SIZE and WRITE_SIZE in KiB summed over the three
+# dispatches, each scaled by bytes / counter reading of the 1 GiB device copy in the same run.
+PMC_SDF_KIB = {"fetch": 8702260.0 + 15652900.0 + 6637.81, "write": 138589.0 + 79480.2 + 18300.3}
+PMC_COPY_KIB = {"fetch": 524308.0, "write": 1048576.0}
+PMC_SDF_L2_HIT = 603713610.0 / (603713610.0 + 220420489.0)    # TCC_HIT_sum / (HIT + MISS), three launches
 COPY_BYTES = 1 << 30
 
 
@@ -459,7 +460,7 @@ def main():
     roof = None
     if b_vox is not None:
         achieved = b_vox * nvox / (sdf_avg / 1000.0) / 1e9
-        kname = "kSdfGridCoop" if args.traversal == "wave_coop" else "kSdfGrid"
+        kname = ("kSdfGridCoop+kSdfSplit+kSdfSplitFinish" if args.traversal == "wave_coop" else "kSdfGrid")
         traffic = pmc_traffic_bytes() if args.traversal == "wave_coop" else None
         roof = {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
