@@ -25,6 +25,10 @@
 #include <cfloat>
 #include <cstdint>
 
+// v_writelane_b32: `old` with lane `lane` replaced by the wave-uniform `value` (the compiler's
+// intrinsic, reached by name: clang has no builtin for it)
+__device__ int mi_writeLane(int value, int lane, int old) __asm("llvm.amdgcn.writelane");
+
 namespace mi {
 namespace {
 
@@ -89,13 +93,17 @@ inline int paddedGrid(int nLogical)
 }
 
 // Box::getDistanceSq(pt) (source/MRMesh/MRBox.h:207-220): the per-dim term is the
-// distance to the nearer face outside the box, zero inside; summed x, y, z.
+// distance to the nearer face outside the box, zero inside; summed x, y, z.  The term is
+// formed as clamp(p, mn, mx) - p (one med3 and one subtraction): below the box it is
+// mn - p, above it mx - p = -(p - mx) exactly, inside p - p = 0, so its square equals the
+// reference's max(mn - p, p - mx, 0)^2 bit for bit.  Requires mn <= mx; an empty slot is
+// the point box at +inf (distance +inf).
 __device__ __forceinline__ float boxDistSq(float mnx, float mny, float mnz, float mxx, float mxy,
                                            float mxz, float px, float py, float pz)
 {
-    const float tx = fmaxf(fmaxf(mnx - px, px - mxx), 0.0f);
-    const float ty = fmaxf(fmaxf(mny - py, py - mxy), 0.0f);
-    const float tz = fmaxf(fmaxf(mnz - pz, pz - mxz), 0.0f);
+    const float tx = __builtin_amdgcn_fmed3f(px, mnx, mxx) - px;
+    const float ty = __builtin_amdgcn_fmed3f(py, mny, mxy) - py;
+    const float tz = __builtin_amdgcn_fmed3f(pz, mnz, mxz) - pz;
     return tx * tx + ty * ty + tz * tz;
 }
 
@@ -188,6 +196,20 @@ __device__ __forceinline__ void closestPointD(const double p[3], const double a[
     pr[1] = a[1] + v * aby + w * acy;
     pr[2] = a[2] + v * abz + w * acz;
     ba = v; bb = w;
+}
+
+// exact squared distance of (px, py, pz) to a leaf's triangle: the FP64 closest point rounded to
+// float, then the float sum of squares (the leaf test of findProjectionSubtree)
+__device__ __forceinline__ float leafDistSq(const Leaf& L, float px, float py, float pz)
+{
+    const double pd[3] = { px, py, pz };
+    const double ad[3] = { L.a[0], L.a[1], L.a[2] };
+    const double bd[3] = { L.b[0], L.b[1], L.b[2] };
+    const double cd[3] = { L.c[0], L.c[1], L.c[2] };
+    double pr[3], ba, bb;
+    closestPointD(pd, ad, bd, cd, pr, ba, bb);
+    const float dx = (float)pr[0] - px, dy = (float)pr[1] - py, dz = (float)pr[2] - pz;
+    return dx * dx + dy * dy + dz * dz;
 }
 
 // triangleSolidAngle (source/MRMesh/MRDipole.cpp:81-92), det as MRMatrix3.h:226-232
@@ -468,25 +490,25 @@ __device__ __forceinline__ float laneValue(float v, int lane)
 // candidate, so seeding a walk with it leaves the result unchanged while every lane prunes from the
 // first node on -- the walks of a wave whose lanes are far apart no longer follow one lane's order
 // with the other lanes' limits still open.
+// Per-lane refs: plain (vector) loads, kept apart from the scalar-load pointers of the walks so the
+// compiler does not move the walks' record addresses into vector registers as well.
 __device__ __forceinline__ float greedyLeafDistSq(const MeshDev& m, float px, float py, float pz)
 {
-    const double pd[3] = { px, py, pz };
     int cur = m.rootRef;
     while (cur >= 0) {
-        const float4 n0 = cload(m.dnode, 4 * cur), n1 = cload(m.dnode, 4 * cur + 1);
-        const float4 n2 = cload(m.dnode, 4 * cur + 2), n3 = cload(m.dnode, 4 * cur + 3);
+        const float4 n0 = m.dnode[4 * cur], n1 = m.dnode[4 * cur + 1];
+        const float4 n2 = m.dnode[4 * cur + 2], n3 = m.dnode[4 * cur + 3];
         const float d1 = boxDistSq(n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, px, py, pz);
         const float d2 = boxDistSq(n1.z, n1.w, n2.x, n2.y, n2.z, n2.w, px, py, pz);
         cur = d1 <= d2 ? __float_as_int(n3.x) : __float_as_int(n3.y);
     }
-    const Leaf L = loadLeaf(m.tri, ~cur);
-    const double ad[3] = { L.a[0], L.a[1], L.a[2] };
-    const double bd[3] = { L.b[0], L.b[1], L.b[2] };
-    const double cd[3] = { L.c[0], L.c[1], L.c[2] };
-    double pr[3], ba, bb;
-    closestPointD(pd, ad, bd, cd, pr, ba, bb);
-    const float dx = (float)pr[0] - px, dy = (float)pr[1] - py, dz = (float)pr[2] - pz;
-    return dx * dx + dy * dy + dz * dz;
+    const float4 t0 = m.tri[3 * (size_t)~cur], t1 = m.tri[3 * (size_t)~cur + 1], t2 = m.tri[3 * (size_t)~cur + 2];
+    Leaf L;
+    L.a[0] = t0.x; L.a[1] = t0.y; L.a[2] = t0.z;
+    L.b[0] = t0.w; L.b[1] = t1.x; L.b[2] = t1.y;
+    L.c[0] = t1.z; L.c[1] = t1.w; L.c[2] = t2.x;
+    L.face = __float_as_int(t2.y);
+    return leafDistSq(L, px, py, pz);
 }
 
 // Minimum squared distance of each lane's point to the mesh below `up` (every leaf whose box is
@@ -544,6 +566,35 @@ __device__ __forceinline__ int rankSlots(const uint64_t b[4], const float d[4], 
     return cnt;
 }
 
+// lanes with a < b (ordered compare) as a wave mask, straight from the compare
+__device__ __forceinline__ uint64_t ltMask(float a, float b) { return __builtin_amdgcn_fcmpf(a, b, 4); }
+
+// Stack of a wave-cooperative distance walk.  The node ref of entry e is held by lane e of one
+// VGPR (writelane / readlane move wave-uniform values without memory), and each lane's box
+// distance of the entry sits in LDS at d[e][lane]: a pop reads one value per lane instead of
+// reloading the box and recomputing its distance.  Only the distance's top 16 bits are kept
+// (5 KB per wave at depth 24, so LDS does not limit occupancy): for a non-negative float that is
+// a lower bound within 2^-7 relative, so a pop can wake a lane whose box lies just past its best
+// -- that lane then finds nothing closer (slot and leaf tests are exact) -- and never sleeps a
+// lane whose box is closer.
+template <int D>
+struct DistStack {
+    uint16_t d[quadStack(D)][64];
+};
+__device__ __forceinline__ uint16_t stackDist(float d) { return (uint16_t)(__float_as_uint(d) >> 16); }
+__device__ __forceinline__ float stackDist(uint16_t s) { return __uint_as_float((uint32_t)s << 16); }
+static_assert(quadStack(32) <= 64, "stack entries are held one per lane");
+
+// The leaf record through scalar loads, issued before any divergent branch: the compiler would
+// otherwise sink the loads into the active lanes' branch and turn them into 64 vector loads.
+__device__ __forceinline__ Leaf loadLeafUniform(const float4* __restrict__ tri, int k)
+{
+    const Leaf L = loadLeaf(tri, k);
+    asm volatile("" ::"s"(L.a[0]), "s"(L.a[1]), "s"(L.a[2]), "s"(L.b[0]), "s"(L.b[1]), "s"(L.b[2]),
+                 "s"(L.c[0]), "s"(L.c[1]), "s"(L.c[2]));
+    return L;
+}
+
 // Squared distance to the nearest leaf (calcDistanceSq / findProjectionSubtree distance part,
 // source/MRCuda/MRCudaFastWindingNumber.cu:65-133) for each lane, on a wave-shared walk over the
 // 4-wide records: the minimum does not depend on the visit order, so it equals the per-lane
@@ -552,40 +603,34 @@ __device__ __forceinline__ int rankSlots(const uint64_t b[4], const float d[4], 
 // The walk of coopDistance from node `cur` (box distance `curd`), for lanes whose best so far is
 // `best`; after `budget` visited steps (0 = no limit) it stops and sets `stopped` (wave-uniform),
 // returning the best found so far -- a real candidate, so a later walk can finish from it.
+// A slot box lies inside its parent's box, so its distance is never below the parent's (the float
+// box distance is monotone in the box): a lane that passes a slot test passed the parent's.
 template <int D>
 __device__ __forceinline__ float coopDistanceWalk(const MeshDev& m, float px, float py, float pz, float best,
-                                                  int cur, float curd, float4* stk, int budget, bool& stopped)
+                                                  int cur, float curd, DistStack<D>* stk, int budget, bool& stopped)
 {
+    const int lane = threadIdx.x & 63;
     stopped = false;
     int sp = 0, steps = 0;
+    int refs = 0;      // lane e: node ref of stack entry e
     bool have = true;
     for (;;) {
         if (!have) {
             if (sp == 0) break;
             --sp;
-            const float4 e0 = stk[2 * sp], e1 = stk[2 * sp + 1];
-            cur = __float_as_int(e0.x);
-            curd = boxDistSq(e0.y, e0.z, e0.w, e1.x, e1.y, e1.z, px, py, pz);
+            cur = __builtin_amdgcn_readlane(refs, sp);
+            curd = stackDist(stk->d[sp][lane]);
         }
         have = false;
-        const bool act = curd < best;
-        if (!waveBallot(act)) continue;
+        if (!ltMask(curd, best)) continue;
         if (budget && ++steps > budget) {
             stopped = true;
             break;
         }
-        cur = uniform(cur);
         if (cur < 0) {
-            const Leaf L = loadLeaf(m.tri, ~cur);
-            if (act) {
-                const double pd[3] = { px, py, pz };
-                const double ad[3] = { L.a[0], L.a[1], L.a[2] };
-                const double bd[3] = { L.b[0], L.b[1], L.b[2] };
-                const double cd[3] = { L.c[0], L.c[1], L.c[2] };
-                double pr[3], ba, bb;
-                closestPointD(pd, ad, bd, cd, pr, ba, bb);
-                const float dx = (float)pr[0] - px, dy = (float)pr[1] - py, dz = (float)pr[2] - pz;
-                const float d = dx * dx + dy * dy + dz * dz;
+            const Leaf L = loadLeafUniform(m.tri, ~cur);
+            if (curd < best) {
+                const float d = leafDistSq(L, px, py, pz);
                 if (d < best) best = d;
             }
             continue;
@@ -596,21 +641,34 @@ __device__ __forceinline__ float coopDistanceWalk(const MeshDev& m, float px, fl
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             d[i] = slotDistSq(Q, i, px, py, pz);
-            b[i] = waveBallot(act && d[i] < best);
+            b[i] = ltMask(d[i], best);
         }
-        if (!(b[0] | b[1] | b[2] | b[3])) continue;
-        int rank[4];
-        const int cnt = rankSlots(b, d, rank);
+        const uint64_t any = b[0] | b[1] | b[2] | b[3];
+        if (!any) continue;
+        // order keys: the lowest active lane's distance bits (non-negative floats order like their
+        // bits, all below 2^31) with the slot index in the two low bits; 0 for inactive slots
+        const int lead = __builtin_ctzll(any);
+        uint32_t k[4];
+        int cnt = 0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            k[i] = b[i] ? ((__float_as_uint(laneValue(d[i], lead)) & ~3u) | (uint32_t)i) : 0u;
+            cnt += b[i] ? 1 : 0;
+        }
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             if (!b[i]) continue;
-            if (rank[i] == 0) {
+            int farther = 0;     // active slots farther than slot i (an inactive key 0 never counts)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (j != i) farther += (int)((k[i] - k[j]) >> 31);
+            if (farther == cnt - 1) {
                 cur = Q.ref[i];
                 curd = d[i];
             } else {
-                const int pos = sp + cnt - 1 - rank[i];
-                stk[2 * pos] = make_float4(__int_as_float(Q.ref[i]), Q.mn[i][0], Q.mn[i][1], Q.mn[i][2]);
-                stk[2 * pos + 1] = make_float4(Q.mx[i][0], Q.mx[i][1], Q.mx[i][2], 0.f);
+                const int pos = sp + farther;
+                refs = mi_writeLane(Q.ref[i], pos, refs);
+                stk->d[pos][lane] = stackDist(d[i]);
             }
         }
         sp += cnt - 1;
@@ -626,7 +684,7 @@ __device__ __forceinline__ float coopDistanceWalk(const MeshDev& m, float px, fl
 // the lowest active lane) continues, the other active ones are pushed farthest first.
 template <int D>
 __device__ __forceinline__ float coopDistance(const MeshDev& m, float px, float py, float pz, float up,
-                                              float4* stk, int budget = 0, bool* stopped = nullptr)
+                                              DistStack<D>* stk, int budget = 0, bool* stopped = nullptr)
 {
     float best = up;
     if (stopped) *stopped = false;
@@ -1101,7 +1159,7 @@ __global__ __launch_bounds__(kBlock) void kSdfGridCoop(MeshDev m, int dx, int dy
                                                        uint32_t* __restrict__ rerunCount,
                                                        SplitScratch sc, float* __restrict__ out)
 {
-    __shared__ float4 stkMem[kWaves][2 * quadStack(D)];
+    __shared__ DistStack<D> stkMem[kWaves];
     const int lb = logicalBlock(nLogical);
     if (lb >= nLogical) return;
     const int lane = threadIdx.x & 63;
@@ -1116,7 +1174,7 @@ __global__ __launch_bounds__(kBlock) void kSdfGridCoop(MeshDev m, int dx, int dy
     float lim = o.maxDistSq;
     if (bound && valid) { const float b = bound[idx]; lim = b < lim ? b : lim; }
     bool stopped = false;
-    const float resSq = coopDistance<D>(m, px, py, pz, valid ? lim : -1.f, stkMem[w], sc.list ? sc.budget : 0,
+    const float resSq = coopDistance<D>(m, px, py, pz, valid ? lim : -1.f, &stkMem[w], sc.list ? sc.budget : 0,
                                         &stopped);
     if (stopped) {
         // hand the brick to the split pass with every lane's best so far
@@ -1130,7 +1188,7 @@ __global__ __launch_bounds__(kBlock) void kSdfGridCoop(MeshDev m, int dx, int dy
     }
     const bool again = rerun && valid && !(resSq < lim) && lim < o.maxDistSq;
     if (rerun) appendWave(again, (uint32_t)idx, rerun, rerunCount);
-    sdfStore(m, px, py, pz, valid && !again, resSq, o, reinterpret_cast<uint4*>(stkMem[w]), out, idx);
+    sdfStore(m, px, py, pz, valid && !again, resSq, o, reinterpret_cast<uint4*>(&stkMem[w]), out, idx);
 }
 
 // Second launch of the split form: persistent waves take (handed-over brick, cut subtree) pairs and
@@ -1140,7 +1198,7 @@ template <int D>
 __global__ __launch_bounds__(kBlock) void kSdfSplit(MeshDev m, int dx, int dy, int zBegin, int zEnd, int nbx,
                                                       int nby, int nbz, Xf xf, SplitScratch sc)
 {
-    __shared__ float4 stkMem[kWaves][2 * quadStack(D)];
+    __shared__ DistStack<D> stkMem[kWaves];
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
     const uint32_t nHard = sc.counters[0] < sc.cap ? sc.counters[0] : sc.cap;
@@ -1162,7 +1220,7 @@ __global__ __launch_bounds__(kBlock) void kSdfSplit(MeshDev m, int dx, int dy, i
         const CutEntry E = loadCut(m, c);
         const float cd = boxDistSq(E.mn[0], E.mn[1], E.mn[2], E.mx[0], E.mx[1], E.mx[2], px, py, pz);
         bool st;
-        const float r = coopDistanceWalk<D>(m, px, py, pz, best, E.ref, cd, stkMem[w], 0, st);
+        const float r = coopDistanceWalk<D>(m, px, py, pz, best, E.ref, cd, &stkMem[w], 0, st);
         if (valid && r < best) atomicMin(&sc.best[(size_t)h * 64 + lane], __float_as_uint(r));
     }
 }
@@ -1200,7 +1258,7 @@ __global__ __launch_bounds__(kBlock) void kSdfVoxelsCoop(MeshDev m, int dx, int 
                                                          int nLogical, Xf xf, DistOpts o,
                                                          float* __restrict__ out)
 {
-    __shared__ float4 stkMem[kWaves][2 * quadStack(D)];
+    __shared__ DistStack<D> stkMem[kWaves];
     const int lb = logicalBlock(nLogical);
     if (lb >= nLogical) return;
     const int64_t i = (int64_t)lb * kBlock + threadIdx.x;
@@ -1213,8 +1271,8 @@ __global__ __launch_bounds__(kBlock) void kSdfVoxelsCoop(MeshDev m, int dx, int 
     float px = (float)(rem % dx), py = (float)(rem / dx), pz = (float)z;
     xfApply(xf, px, py, pz);
     const int w = threadIdx.x >> 6;
-    const float resSq = coopDistance<D>(m, px, py, pz, valid ? o.maxDistSq : -1.f, stkMem[w]);
-    sdfStore(m, px, py, pz, valid, resSq, o, reinterpret_cast<uint4*>(stkMem[w]), out, idx);
+    const float resSq = coopDistance<D>(m, px, py, pz, valid ? o.maxDistSq : -1.f, &stkMem[w]);
+    sdfStore(m, px, py, pz, valid, resSq, o, reinterpret_cast<uint4*>(&stkMem[w]), out, idx);
 }
 
 template <int D>
@@ -1289,7 +1347,11 @@ __global__ __launch_bounds__(kBlock) void kBatch(const BatchItem* __restrict__ i
                                                  DistOpts o, float up, unsigned long long* __restrict__ tickets,
                                                  int xcdSplit)
 {
-    __shared__ float4 stkMem[kWaves][3 * quadStack(D)];
+    union WaveStack {
+        DistStack<D> dist;
+        float4 proj[3 * quadStack(D)];
+    };
+    __shared__ WaveStack stkMem[kWaves];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int parts = xcdSplit ? kXcds : 1;
     int part = xcdSplit ? (int)(blockIdx.x % kXcds) : 0;
@@ -1334,10 +1396,11 @@ __global__ __launch_bounds__(kBlock) void kBatch(const BatchItem* __restrict__ i
             xfApply(it.xf, px, py, pz);
             float lim = o.maxDistSq;
             if (it.capSdf && valid) { const float c = it.capSdf[idx]; lim = c < lim ? c : lim; }
-            const float resSq = coopDistance<D>(it.m, px, py, pz, valid ? lim : -1.f, stkMem[w]);
+            const float resSq = coopDistance<D>(it.m, px, py, pz, valid ? lim : -1.f, &stkMem[w].dist);
             const bool again = it.rerunVox && valid && !(resSq < lim) && lim < o.maxDistSq;
             if (it.rerunVox) appendWave(again, (uint32_t)idx, it.rerunVox, it.rerunCount);
-            sdfStore(it.m, px, py, pz, valid && !again, resSq, o, reinterpret_cast<uint4*>(stkMem[w]), it.sdf, idx);
+            sdfStore(it.m, px, py, pz, valid && !again, resSq, o, reinterpret_cast<uint4*>(stkMem[w].proj), it.sdf,
+                     idx);
         } else {
             const int64_t i = (lu - it.gridUnits) * 64 + lane;
             const bool valid = i < it.npts;
@@ -1349,7 +1412,7 @@ __global__ __launch_bounds__(kBlock) void kBatch(const BatchItem* __restrict__ i
                 if (it.capProj) { const float c = it.capProj[q]; lim = c < up ? c : up; }
             }
             BestHit h;
-            coopProject<D>(it.m, px, py, pz, lim, valid, stkMem[w], h);
+            coopProject<D>(it.m, px, py, pz, lim, valid, stkMem[w].proj, h);
             const bool again = it.rerunPts && valid && h.face < 0 && lim < up;
             if (it.rerunPts) appendWave(again, (uint32_t)q, it.rerunPts, it.rerunCount + 1);
             if (valid && !again) {
@@ -1401,7 +1464,7 @@ hipError_t dispatchDepth(int D, Fn&& fn)
 namespace {
 // 4-wide record of internal node j (layout read by loadQuad): slots 0,1 = the left child's
 // children, 2,3 = the right child's; a leaf child takes slot 0 / 2 itself and leaves slot 1 / 3
-// empty (inverted box: never closer than any limit)
+// empty (the point box at +inf: distance +inf, never below any limit)
 __global__ void kBuildQuads(const float4* __restrict__ dnode, int32_t ni, float4* __restrict__ dquad)
 {
     const int32_t j = blockIdx.x * 256 + threadIdx.x;
@@ -1426,7 +1489,7 @@ __global__ void kBuildQuads(const float4* __restrict__ dnode, int32_t ni, float4
             ref[2 * side + 1] = __float_as_int(c3.y);
         } else {
             for (int k = 0; k < 6; ++k) bx[2 * side][k] = cb[side][k];
-            for (int k = 0; k < 3; ++k) { bx[2 * side + 1][k] = INFINITY; bx[2 * side + 1][3 + k] = -INFINITY; }
+            for (int k = 0; k < 6; ++k) bx[2 * side + 1][k] = INFINITY;
             ref[2 * side] = c;
             ref[2 * side + 1] = 0;
         }
