@@ -577,22 +577,43 @@ __device__ __forceinline__ uint64_t ltMask(float a, float b) { return __builtin_
 // a lower bound within 2^-7 relative, so a pop can wake a lane whose box lies just past its best
 // -- that lane then finds nothing closer (slot and leaf tests are exact) -- and never sleeps a
 // lane whose box is closer.
+//
+// Leaf tests are queued rather than run at the visit: a leaf visit appends (leaf, lane) for its
+// active lanes, and a full queue is run one test per lane (flushLeaves), so the FP64 closest point
+// runs on all 64 lanes instead of the few whose box was close.  Results meet in best[] by LDS
+// atomic minimum.  Pruning between flushes uses a best that may lag behind, which only adds
+// visits: every leaf closer than the final best is still tested, so the minimum is unchanged.
 template <int D>
 struct DistStack {
     uint16_t d[quadStack(D)][64];
+    int32_t leaf[64];       // queued leaf tests: leaf slot
+    uint8_t owner[64];      //   and the lane that asked
+    uint32_t best[64];      // each lane's best squared distance (float bits; >= 0 for searching lanes)
 };
 __device__ __forceinline__ uint16_t stackDist(float d) { return (uint16_t)(__float_as_uint(d) >> 16); }
 __device__ __forceinline__ float stackDist(uint16_t s) { return __uint_as_float((uint32_t)s << 16); }
 static_assert(quadStack(32) <= 64, "stack entries are held one per lane");
 
-// The leaf record through scalar loads, issued before any divergent branch: the compiler would
-// otherwise sink the loads into the active lanes' branch and turn them into 64 vector loads.
-__device__ __forceinline__ Leaf loadLeafUniform(const float4* __restrict__ tri, int k)
+// Runs the `qn` queued leaf tests, one per lane: lane j takes entry j, fetches its leaf (a vector
+// load: entries name different leaves) and its owner's point (cross-lane read), and lowers the
+// owner's best.  Returns this lane's best afterwards.
+template <int D>
+__device__ __forceinline__ float flushLeaves(const MeshDev& m, float px, float py, float pz, DistStack<D>* stk, int qn)
 {
-    const Leaf L = loadLeaf(tri, k);
-    asm volatile("" ::"s"(L.a[0]), "s"(L.a[1]), "s"(L.a[2]), "s"(L.b[0]), "s"(L.b[1]), "s"(L.b[2]),
-                 "s"(L.c[0]), "s"(L.c[1]), "s"(L.c[2]));
-    return L;
+    const int lane = threadIdx.x & 63;
+    const int owner = stk->owner[lane];
+    const float ox = __shfl(px, owner), oy = __shfl(py, owner), oz = __shfl(pz, owner);
+    if (lane < qn) {
+        const int k = stk->leaf[lane];
+        const float4 t0 = m.tri[3 * (size_t)k], t1 = m.tri[3 * (size_t)k + 1], t2 = m.tri[3 * (size_t)k + 2];
+        Leaf L;
+        L.a[0] = t0.x; L.a[1] = t0.y; L.a[2] = t0.z;
+        L.b[0] = t0.w; L.b[1] = t1.x; L.b[2] = t1.y;
+        L.c[0] = t1.z; L.c[1] = t1.w; L.c[2] = t2.x;
+        const float d = leafDistSq(L, ox, oy, oz);
+        atomicMin(&stk->best[owner], __float_as_uint(d));
+    }
+    return __uint_as_float(stk->best[lane]);
 }
 
 // Squared distance to the nearest leaf (calcDistanceSq / findProjectionSubtree distance part,
@@ -611,9 +632,10 @@ __device__ __forceinline__ float coopDistanceWalk(const MeshDev& m, float px, fl
 {
     const int lane = threadIdx.x & 63;
     stopped = false;
-    int sp = 0, steps = 0;
+    int sp = 0, steps = 0, qn = 0;
     int refs = 0;      // lane e: node ref of stack entry e
     bool have = true;
+    stk->best[lane] = __float_as_uint(best);
     for (;;) {
         if (!have) {
             if (sp == 0) break;
@@ -622,17 +644,25 @@ __device__ __forceinline__ float coopDistanceWalk(const MeshDev& m, float px, fl
             curd = stackDist(stk->d[sp][lane]);
         }
         have = false;
-        if (!ltMask(curd, best)) continue;
+        uint64_t act = ltMask(curd, best);
+        if (!act) continue;
         if (budget && ++steps > budget) {
             stopped = true;
             break;
         }
         if (cur < 0) {
-            const Leaf L = loadLeafUniform(m.tri, ~cur);
-            if (curd < best) {
-                const float d = leafDistSq(L, px, py, pz);
-                if (d < best) best = d;
+            if (qn + __popcll(act) > 64) {
+                best = flushLeaves<D>(m, px, py, pz, stk, qn);
+                qn = 0;
+                act = ltMask(curd, best);
             }
+            if (curd < best) {
+                const int r = __builtin_amdgcn_mbcnt_hi((uint32_t)(act >> 32),
+                                                        __builtin_amdgcn_mbcnt_lo((uint32_t)act, 0u));
+                stk->leaf[qn + r] = ~cur;
+                stk->owner[qn + r] = (uint8_t)lane;
+            }
+            qn += __popcll(act);
             continue;
         }
         const Quad Q = loadQuad(m, cur);
@@ -674,6 +704,7 @@ __device__ __forceinline__ float coopDistanceWalk(const MeshDev& m, float px, fl
         sp += cnt - 1;
         have = true;
     }
+    if (qn) best = flushLeaves<D>(m, px, py, pz, stk, qn);
     return best;
 }
 
