@@ -212,6 +212,16 @@ __device__ __forceinline__ float leafDistSq(const Leaf& L, float px, float py, f
     return dx * dx + dy * dy + dz * dz;
 }
 
+// The leaf record through scalar loads, issued before any divergent branch: the compiler would
+// otherwise sink the loads into the active lanes' branch and turn them into 64 vector loads.
+__device__ __forceinline__ Leaf loadLeafUniform(const float4* __restrict__ tri, int k)
+{
+    const Leaf L = loadLeaf(tri, k);
+    asm volatile("" ::"s"(L.a[0]), "s"(L.a[1]), "s"(L.a[2]), "s"(L.b[0]), "s"(L.b[1]), "s"(L.b[2]),
+                 "s"(L.c[0]), "s"(L.c[1]), "s"(L.c[2]));
+    return L;
+}
+
 // triangleSolidAngle (source/MRMesh/MRDipole.cpp:81-92), det as MRMatrix3.h:226-232
 __device__ __forceinline__ float solidAngle(float px, float py, float pz, const Leaf& L)
 {
@@ -544,28 +554,6 @@ __device__ __forceinline__ float slotDistSq(const Quad& Q, int i, float px, floa
     return boxDistSq(Q.mn[i][0], Q.mn[i][1], Q.mn[i][2], Q.mx[i][0], Q.mx[i][1], Q.mx[i][2], px, py, pz);
 }
 
-// Ranks of the active slots (b[i] != 0) by the lead lane's distance, ties by slot index: 0 = the
-// slot the walk continues with, the others go on the stack farthest first.
-__device__ __forceinline__ int rankSlots(const uint64_t b[4], const float d[4], int rank[4])
-{
-    const uint64_t any = b[0] | b[1] | b[2] | b[3];
-    const int lead = __builtin_ctzll(any);
-    uint32_t k[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) k[i] = b[i] ? __float_as_uint(laneValue(d[i], lead)) : 0xffffffffu;
-    int cnt = 0;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        cnt += b[i] != 0;
-        int r = 0;
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-            if (j != i && b[j] && (k[j] < k[i] || (k[j] == k[i] && j < i))) ++r;
-        rank[i] = r;
-    }
-    return cnt;
-}
-
 // lanes with a < b (ordered compare) as a wave mask, straight from the compare
 __device__ __forceinline__ uint64_t ltMask(float a, float b) { return __builtin_amdgcn_fcmpf(a, b, 4); }
 
@@ -800,45 +788,58 @@ __device__ __forceinline__ float coopWinding(const MeshDev& m, float qx, float q
 // prefix `curKey`), improving each lane's (res.distSq, bestKey); after `budget` visited steps (0 =
 // no limit) it stops and sets `stopped` (wave-uniform).
 template <int D>
+struct ProjStack {
+    uint16_t d[quadStack(D)][64];     // each lane's box distance of entry e (top 16 bits: a lower bound)
+};
+
+// lanes whose (d, key) is below (bd, bk) in the order (distance, key)
+__device__ __forceinline__ uint64_t lexMask(float d, uint32_t key, float bd, uint32_t bk)
+{
+    return ltMask(d, bd) | (__builtin_amdgcn_fcmpf(d, bd, 1) & __builtin_amdgcn_uicmp(key, bk, 36));
+}
+
+template <int D>
 __device__ __forceinline__ void coopProjectWalk(const MeshDev& m, float px, float py, float pz, int cur,
-                                                int curDepth, float curd, uint32_t curKey, float4* stk,
+                                                int curDepth, float curd, uint32_t curKey, ProjStack<D>* stk,
                                                 BestHit& res, uint32_t& bestKey, int budget, bool& stopped)
 {
     stopped = false;
     const int lane = threadIdx.x & 63;
     int sp = 0, steps = 0;
+    // lane e of each: stack entry e's node ref, depth | 0x100 (relation: child of the other child),
+    // and for that relation the 64-lane mask of the entry's own key bit
+    int refs = 0, infos = 0, masksLo = 0, masksHi = 0;
     bool have = true;
     for (;;) {
         if (!have) {
             if (sp == 0) break;
             --sp;
-            const float4 e0 = stk[3 * sp], e1 = stk[3 * sp + 1], e2 = stk[3 * sp + 2];
-            cur = __float_as_int(e0.x);
-            curd = boxDistSq(e0.y, e0.z, e0.w, e1.x, e1.y, e1.z, px, py, pz);
-            const int info = __float_as_int(e1.w);
+            cur = __builtin_amdgcn_readlane(refs, sp);
+            curd = stackDist(stk->d[sp][lane]);
+            const int info = __builtin_amdgcn_readlane(infos, sp);
             curDepth = info & 0xff;                        // >= 1
             const uint32_t bit = 1u << (32 - curDepth);
             if (info & 0x100) {
                 const uint32_t pbit = bit << 1;
                 curKey = (curKey & ~(pbit - 1u)) ^ pbit;
-                const uint64_t mask = (uint64_t)__float_as_uint(e2.x) | ((uint64_t)__float_as_uint(e2.y) << 32);
+                const uint64_t mask = (uint64_t)(uint32_t)__builtin_amdgcn_readlane(masksLo, sp) |
+                                      ((uint64_t)(uint32_t)__builtin_amdgcn_readlane(masksHi, sp) << 32);
                 if ((mask >> lane) & 1ull) curKey |= bit;
             } else {
                 curKey = (curKey & ~(bit - 1u)) ^ bit;
             }
         }
         have = false;
-        const bool act = curd < res.distSq || (curd == res.distSq && curKey < bestKey);
-        if (!waveBallot(act)) continue;
+        // popped distances are lower bounds: the test may keep a lane whose box lies just past its
+        // best (its slot and leaf tests are exact), never drops one that could still win
+        if (!lexMask(curd, curKey, res.distSq, bestKey)) continue;
         if (budget && ++steps > budget) {
             stopped = true;
             break;
         }
-        cur = uniform(cur);
-        curDepth = uniform(curDepth);
         if (cur < 0) {
-            const Leaf L = loadLeaf(m.tri, ~cur);
-            if (act) {
+            const Leaf L = loadLeafUniform(m.tri, ~cur);
+            if (curd < res.distSq || (curd == res.distSq && curKey < bestKey)) {
                 const double pd[3] = { px, py, pz };
                 const double ad[3] = { L.a[0], L.a[1], L.a[2] };
                 const double bd[3] = { L.b[0], L.b[1], L.b[2] };
@@ -877,19 +878,34 @@ __device__ __forceinline__ void coopProjectWalk(const MeshDev& m, float px, floa
         key[1] = kA | (nearA0 ? bit2 : 0u);
         key[2] = Q.leafB ? kB : (kB | (nearB0 ? 0u : bit2));
         key[3] = kB | (nearB0 ? bit2 : 0u);
+        // a slot's (distance, key) is never below its parent's, so a slot that passes implies that
+        // the binary walk would have reached it
         uint64_t b[4];
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
-            b[i] = waveBallot(act && (d[i] < res.distSq || (d[i] == res.distSq && key[i] < bestKey)));
-        if (!(b[0] | b[1] | b[2] | b[3])) continue;
-        int rank[4];
-        const int cnt = rankSlots(b, d, rank);
-        const int baseDepth = curDepth;
-        // side (0: under the left child, 1: right) of the slot at each rank
-        uint32_t sideAt = 0;
+        for (int i = 0; i < 4; ++i) b[i] = lexMask(d[i], key[i], res.distSq, bestKey);
+        const uint64_t any = b[0] | b[1] | b[2] | b[3];
+        if (!any) continue;
+        // order keys as in coopDistanceWalk: lowest active lane's distance bits, slot index below
+        const int lead = __builtin_ctzll(any);
+        uint32_t k[4];
+        int cnt = 0;
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
+        for (int i = 0; i < 4; ++i) {
+            k[i] = b[i] ? ((__float_as_uint(laneValue(d[i], lead)) & ~3u) | (uint32_t)i) : 0u;
+            cnt += b[i] ? 1 : 0;
+        }
+        int rank[4];
+        uint32_t sideAt = 0;      // side (0: under the left child, 1: right) of the slot at each rank
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            int farther = 0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (j != i) farther += (int)((k[i] - k[j]) >> 31);
+            rank[i] = cnt - 1 - farther;
             if (b[i]) sideAt |= (uint32_t)(i >> 1) << rank[i];
+        }
+        const int baseDepth = curDepth;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             if (!b[i]) continue;
@@ -906,15 +922,16 @@ __device__ __forceinline__ void coopProjectWalk(const MeshDev& m, float px, floa
             // relation to the slot handled just before this one
             const int prevSide = (int)((sideAt >> (rank[i] - 1)) & 1u);
             int info = depthI;
-            uint64_t mask = 0;
+            const int pos = sp + cnt - 1 - rank[i];
             if (prevSide != side && !leafSide) {
                 info |= 0x100;
-                mask = waveBallot((key[i] & bit2) != 0u);
+                const uint64_t mask = __builtin_amdgcn_uicmp(key[i] & bit2, 0u, 33);
+                masksLo = mi_writeLane((int)(uint32_t)mask, pos, masksLo);
+                masksHi = mi_writeLane((int)(uint32_t)(mask >> 32), pos, masksHi);
             }
-            const int pos = sp + cnt - 1 - rank[i];
-            stk[3 * pos] = make_float4(__int_as_float(Q.ref[i]), Q.mn[i][0], Q.mn[i][1], Q.mn[i][2]);
-            stk[3 * pos + 1] = make_float4(Q.mx[i][0], Q.mx[i][1], Q.mx[i][2], __int_as_float(info));
-            stk[3 * pos + 2] = make_float4(__uint_as_float((uint32_t)mask), __uint_as_float((uint32_t)(mask >> 32)), 0.f, 0.f);
+            refs = mi_writeLane(Q.ref[i], pos, refs);
+            infos = mi_writeLane(info, pos, infos);
+            stk->d[pos][lane] = stackDist(d[i]);
         }
         sp += cnt - 1;
         have = true;
@@ -923,7 +940,7 @@ __device__ __forceinline__ void coopProjectWalk(const MeshDev& m, float px, floa
 
 template <int D>
 __device__ __forceinline__ void coopProject(const MeshDev& m, float px, float py, float pz, float up, bool valid,
-                                            float4* stk, BestHit& res, uint32_t* bestKeyOut = nullptr,
+                                            ProjStack<D>* stk, BestHit& res, uint32_t* bestKeyOut = nullptr,
                                             int budget = 0, bool* stopped = nullptr)
 {
     res.distSq = valid ? up : -1.f;
@@ -999,7 +1016,7 @@ __global__ __launch_bounds__(kBlock) void kProjectionsCoop(MeshDev m, const floa
                                                            uint32_t* __restrict__ rerunCount,
                                                            SplitScratch sc, ProjRec* __restrict__ out)
 {
-    __shared__ float4 stkMem[kWaves][3 * quadStack(D)];
+    __shared__ ProjStack<D> stkMem[kWaves];
     const int lb = logicalBlock(nLogical);
     if (lb >= nLogical) return;
     const int64_t i = (int64_t)lb * kBlock + threadIdx.x;
@@ -1017,7 +1034,7 @@ __global__ __launch_bounds__(kBlock) void kProjectionsCoop(MeshDev m, const floa
     BestHit h;
     uint32_t key = 0;
     bool stopped = false;
-    coopProject<D>(m, px, py, pz, lim, valid, stkMem[w], h, &key, sc.list ? sc.budget : 0, &stopped);
+    coopProject<D>(m, px, py, pz, lim, valid, &stkMem[w], h, &key, sc.list ? sc.budget : 0, &stopped);
     if (stopped) {
         // hand the wave to the split pass with every lane's (distSq, key) so far
         const int lane = threadIdx.x & 63;
@@ -1066,7 +1083,7 @@ __global__ __launch_bounds__(kBlock) void kProjSplit(MeshDev m, const float* __r
                                                        int hasPtXf, float up, const float* __restrict__ bound,
                                                        SplitScratch sc)
 {
-    __shared__ float4 stkMem[kWaves][3 * quadStack(D)];
+    __shared__ ProjStack<D> stkMem[kWaves];
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
     unsigned long long* best = reinterpret_cast<unsigned long long*>(sc.best);
@@ -1104,7 +1121,7 @@ __global__ __launch_bounds__(kBlock) void kProjSplit(MeshDev m, const float* __r
             node = right ? __float_as_int(n3.y) : __float_as_int(n3.x);
         }
         bool st;
-        coopProjectWalk<D>(m, px, py, pz, E.ref, E.depth, cd, key, stkMem[w], res, bestKey, 0, st);
+        coopProjectWalk<D>(m, px, py, pz, E.ref, E.depth, cd, key, &stkMem[w], res, bestKey, 0, st);
         if (valid && res.face >= 0) {
             const unsigned long long packed = ((unsigned long long)__float_as_uint(res.distSq) << 32) | bestKey;
             if (packed < seed) atomicMin(&best[(size_t)hIdx * 64 + lane], packed);
@@ -1380,7 +1397,8 @@ __global__ __launch_bounds__(kBlock) void kBatch(const BatchItem* __restrict__ i
 {
     union WaveStack {
         DistStack<D> dist;
-        float4 proj[3 * quadStack(D)];
+        ProjStack<D> proj;
+        uint4 fwn[D];
     };
     __shared__ WaveStack stkMem[kWaves];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -1430,8 +1448,7 @@ __global__ __launch_bounds__(kBlock) void kBatch(const BatchItem* __restrict__ i
             const float resSq = coopDistance<D>(it.m, px, py, pz, valid ? lim : -1.f, &stkMem[w].dist);
             const bool again = it.rerunVox && valid && !(resSq < lim) && lim < o.maxDistSq;
             if (it.rerunVox) appendWave(again, (uint32_t)idx, it.rerunVox, it.rerunCount);
-            sdfStore(it.m, px, py, pz, valid && !again, resSq, o, reinterpret_cast<uint4*>(stkMem[w].proj), it.sdf,
-                     idx);
+            sdfStore(it.m, px, py, pz, valid && !again, resSq, o, stkMem[w].fwn, it.sdf, idx);
         } else {
             const int64_t i = (lu - it.gridUnits) * 64 + lane;
             const bool valid = i < it.npts;
@@ -1443,7 +1460,7 @@ __global__ __launch_bounds__(kBlock) void kBatch(const BatchItem* __restrict__ i
                 if (it.capProj) { const float c = it.capProj[q]; lim = c < up ? c : up; }
             }
             BestHit h;
-            coopProject<D>(it.m, px, py, pz, lim, valid, stkMem[w].proj, h);
+            coopProject<D>(it.m, px, py, pz, lim, valid, &stkMem[w].proj, h);
             const bool again = it.rerunPts && valid && h.face < 0 && lim < up;
             if (it.rerunPts) appendWave(again, (uint32_t)q, it.rerunPts, it.rerunCount + 1);
             if (valid && !again) {
