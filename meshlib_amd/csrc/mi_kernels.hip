@@ -212,6 +212,18 @@ __device__ __forceinline__ float leafDistSq(const Leaf& L, float px, float py, f
     return dx * dx + dy * dy + dz * dz;
 }
 
+// The leaf record through vector loads: for per-lane leaf indices.
+__device__ __forceinline__ Leaf loadLeafLanes(const float4* __restrict__ tri, int k)
+{
+    const float4 t0 = tri[3 * (size_t)k], t1 = tri[3 * (size_t)k + 1], t2 = tri[3 * (size_t)k + 2];
+    Leaf L;
+    L.a[0] = t0.x; L.a[1] = t0.y; L.a[2] = t0.z;
+    L.b[0] = t0.w; L.b[1] = t1.x; L.b[2] = t1.y;
+    L.c[0] = t1.z; L.c[1] = t1.w; L.c[2] = t2.x;
+    L.face = __float_as_int(t2.y);
+    return L;
+}
+
 // The leaf record through scalar loads, issued before any divergent branch: the compiler would
 // otherwise sink the loads into the active lanes' branch and turn them into 64 vector loads.
 __device__ __forceinline__ Leaf loadLeafUniform(const float4* __restrict__ tri, int k)
@@ -512,13 +524,7 @@ __device__ __forceinline__ float greedyLeafDistSq(const MeshDev& m, float px, fl
         const float d2 = boxDistSq(n1.z, n1.w, n2.x, n2.y, n2.z, n2.w, px, py, pz);
         cur = d1 <= d2 ? __float_as_int(n3.x) : __float_as_int(n3.y);
     }
-    const float4 t0 = m.tri[3 * (size_t)~cur], t1 = m.tri[3 * (size_t)~cur + 1], t2 = m.tri[3 * (size_t)~cur + 2];
-    Leaf L;
-    L.a[0] = t0.x; L.a[1] = t0.y; L.a[2] = t0.z;
-    L.b[0] = t0.w; L.b[1] = t1.x; L.b[2] = t1.y;
-    L.c[0] = t1.z; L.c[1] = t1.w; L.c[2] = t2.x;
-    L.face = __float_as_int(t2.y);
-    return leafDistSq(L, px, py, pz);
+    return leafDistSq(loadLeafLanes(m.tri, ~cur), px, py, pz);
 }
 
 // Minimum squared distance of each lane's point to the mesh below `up` (every leaf whose box is
@@ -592,13 +598,7 @@ __device__ __forceinline__ float flushLeaves(const MeshDev& m, float px, float p
     const int owner = stk->owner[lane];
     const float ox = __shfl(px, owner), oy = __shfl(py, owner), oz = __shfl(pz, owner);
     if (lane < qn) {
-        const int k = stk->leaf[lane];
-        const float4 t0 = m.tri[3 * (size_t)k], t1 = m.tri[3 * (size_t)k + 1], t2 = m.tri[3 * (size_t)k + 2];
-        Leaf L;
-        L.a[0] = t0.x; L.a[1] = t0.y; L.a[2] = t0.z;
-        L.b[0] = t0.w; L.b[1] = t1.x; L.b[2] = t1.y;
-        L.c[0] = t1.z; L.c[1] = t1.w; L.c[2] = t2.x;
-        const float d = leafDistSq(L, ox, oy, oz);
+        const float d = leafDistSq(loadLeafLanes(m.tri, stk->leaf[lane]), ox, oy, oz);
         atomicMin(&stk->best[owner], __float_as_uint(d));
     }
     return __uint_as_float(stk->best[lane]);
@@ -787,10 +787,41 @@ __device__ __forceinline__ float coopWinding(const MeshDev& m, float qx, float q
 // The walk of coopProject from node `cur` at depth `curDepth` (box distance `curd`, per-lane key
 // prefix `curKey`), improving each lane's (res.distSq, bestKey); after `budget` visited steps (0 =
 // no limit) it stops and sets `stopped` (wave-uniform).
+// Stack and leaf-test queue of a projection walk (see DistStack).  Queued tests carry the owner's
+// key of the leaf; results meet in best[] as (distSq bits << 32 | key) by 64-bit LDS atomic minimum
+// -- the order that decides the record -- and win[] keeps the leaf behind each lane's best.
 template <int D>
 struct ProjStack {
     uint16_t d[quadStack(D)][64];     // each lane's box distance of entry e (top 16 bits: a lower bound)
+    int32_t leaf[64];
+    uint32_t key[64];
+    uint8_t owner[64];
+    unsigned long long best[64];
+    int32_t win[64];                  // leaf of the lane's best found in this walk, -1 before
 };
+
+// Runs the `qn` queued projection leaf tests, one per lane, and refreshes this lane's (bestD, bestKey).
+template <int D>
+__device__ __forceinline__ void flushProjLeaves(const MeshDev& m, float px, float py, float pz, ProjStack<D>* stk,
+                                                int qn, float& bestD, uint32_t& bestKey)
+{
+    const int lane = threadIdx.x & 63;
+    const int owner = stk->owner[lane];
+    const float ox = __shfl(px, owner), oy = __shfl(py, owner), oz = __shfl(pz, owner);
+    unsigned long long packed = ~0ull;
+    int k = 0;
+    if (lane < qn) {
+        k = stk->leaf[lane];
+        const float d = leafDistSq(loadLeafLanes(m.tri, k), ox, oy, oz);
+        packed = ((unsigned long long)__float_as_uint(d) << 32) | stk->key[lane];
+        atomicMin(&stk->best[owner], packed);
+    }
+    // (distSq, key) names one leaf of one lane's order, so exactly the winning entry matches
+    if (lane < qn && stk->best[owner] == packed) stk->win[owner] = k;
+    const unsigned long long b = stk->best[lane];
+    bestD = __uint_as_float((uint32_t)(b >> 32));
+    bestKey = (uint32_t)b;
+}
 
 // lanes whose (d, key) is below (bd, bk) in the order (distance, key)
 __device__ __forceinline__ uint64_t lexMask(float d, uint32_t key, float bd, uint32_t bk)
@@ -798,14 +829,19 @@ __device__ __forceinline__ uint64_t lexMask(float d, uint32_t key, float bd, uin
     return ltMask(d, bd) | (__builtin_amdgcn_fcmpf(d, bd, 1) & __builtin_amdgcn_uicmp(key, bk, 36));
 }
 
+// record = false leaves res's point, face and barycentrics unset (res.face >= 0 still marks an
+// improvement): for callers that keep only (distSq, key).
 template <int D>
 __device__ __forceinline__ void coopProjectWalk(const MeshDev& m, float px, float py, float pz, int cur,
                                                 int curDepth, float curd, uint32_t curKey, ProjStack<D>* stk,
-                                                BestHit& res, uint32_t& bestKey, int budget, bool& stopped)
+                                                BestHit& res, uint32_t& bestKey, int budget, bool& stopped,
+                                                bool record = true)
 {
     stopped = false;
     const int lane = threadIdx.x & 63;
-    int sp = 0, steps = 0;
+    int sp = 0, steps = 0, qn = 0;
+    stk->best[lane] = ((unsigned long long)__float_as_uint(res.distSq) << 32) | bestKey;
+    stk->win[lane] = -1;
     // lane e of each: stack entry e's node ref, depth | 0x100 (relation: child of the other child),
     // and for that relation the 64-lane mask of the entry's own key bit
     int refs = 0, infos = 0, masksLo = 0, masksHi = 0;
@@ -832,31 +868,26 @@ __device__ __forceinline__ void coopProjectWalk(const MeshDev& m, float px, floa
         have = false;
         // popped distances are lower bounds: the test may keep a lane whose box lies just past its
         // best (its slot and leaf tests are exact), never drops one that could still win
-        if (!lexMask(curd, curKey, res.distSq, bestKey)) continue;
+        uint64_t act = lexMask(curd, curKey, res.distSq, bestKey);
+        if (!act) continue;
         if (budget && ++steps > budget) {
             stopped = true;
             break;
         }
         if (cur < 0) {
-            const Leaf L = loadLeafUniform(m.tri, ~cur);
-            if (curd < res.distSq || (curd == res.distSq && curKey < bestKey)) {
-                const double pd[3] = { px, py, pz };
-                const double ad[3] = { L.a[0], L.a[1], L.a[2] };
-                const double bd[3] = { L.b[0], L.b[1], L.b[2] };
-                const double cd[3] = { L.c[0], L.c[1], L.c[2] };
-                double pr[3], ba, bb;
-                closestPointD(pd, ad, bd, cd, pr, ba, bb);
-                const float qx = (float)pr[0], qy = (float)pr[1], qz = (float)pr[2];
-                const float dx = qx - px, dy = qy - py, dz = qz - pz;
-                const float d = dx * dx + dy * dy + dz * dz;
-                if (d < res.distSq || (d == res.distSq && curKey < bestKey)) {
-                    res.distSq = d;
-                    bestKey = curKey;
-                    res.face = L.face;
-                    res.px = qx; res.py = qy; res.pz = qz;
-                    res.a = (float)ba; res.b = (float)bb;
-                }
+            if (qn + __popcll(act) > 64) {
+                flushProjLeaves<D>(m, px, py, pz, stk, qn, res.distSq, bestKey);
+                qn = 0;
+                act = lexMask(curd, curKey, res.distSq, bestKey);
             }
+            if ((act >> lane) & 1ull) {
+                const int r = __builtin_amdgcn_mbcnt_hi((uint32_t)(act >> 32),
+                                                        __builtin_amdgcn_mbcnt_lo((uint32_t)act, 0u));
+                stk->leaf[qn + r] = ~cur;
+                stk->key[qn + r] = curKey;
+                stk->owner[qn + r] = (uint8_t)lane;
+            }
+            qn += __popcll(act);
             continue;
         }
         const Quad Q = loadQuad(m, cur);
@@ -935,6 +966,25 @@ __device__ __forceinline__ void coopProjectWalk(const MeshDev& m, float px, floa
         }
         sp += cnt - 1;
         have = true;
+    }
+    if (qn) flushProjLeaves<D>(m, px, py, pz, stk, qn, res.distSq, bestKey);
+    // the record of the winning leaf (its distance is the one just merged, recomputed identically)
+    const int wl = stk->win[lane];
+    if (wl >= 0) {
+        if (record) {
+            const Leaf L = loadLeafLanes(m.tri, wl);
+            const double pd[3] = { px, py, pz };
+            const double ad[3] = { L.a[0], L.a[1], L.a[2] };
+            const double bd[3] = { L.b[0], L.b[1], L.b[2] };
+            const double cd[3] = { L.c[0], L.c[1], L.c[2] };
+            double pr[3], ba, bb;
+            closestPointD(pd, ad, bd, cd, pr, ba, bb);
+            res.face = L.face;
+            res.px = (float)pr[0]; res.py = (float)pr[1]; res.pz = (float)pr[2];
+            res.a = (float)ba; res.b = (float)bb;
+        } else {
+            res.face = 0;
+        }
     }
 }
 
@@ -1121,7 +1171,7 @@ __global__ __launch_bounds__(kBlock) void kProjSplit(MeshDev m, const float* __r
             node = right ? __float_as_int(n3.y) : __float_as_int(n3.x);
         }
         bool st;
-        coopProjectWalk<D>(m, px, py, pz, E.ref, E.depth, cd, key, &stkMem[w], res, bestKey, 0, st);
+        coopProjectWalk<D>(m, px, py, pz, E.ref, E.depth, cd, key, &stkMem[w], res, bestKey, 0, st, false);
         if (valid && res.face >= 0) {
             const unsigned long long packed = ((unsigned long long)__float_as_uint(res.distSq) << 32) | bestKey;
             if (packed < seed) atomicMin(&best[(size_t)hIdx * 64 + lane], packed);
