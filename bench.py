@@ -36,9 +36,9 @@ def log(*a):
 # `bench.py --steps 1 --warmup 0 --no-cpu --no-learned --no-plain --copy-calibration`
 # (profiles/sdf_grid_coop_k8_pmc.md): FETCH_SIZE and WRITE_SIZE in KiB summed over the three
 # dispatches, each scaled by bytes / counter reading of the 1 GiB device copy in the same run.
-PMC_SDF_KIB = {"fetch": 8702260.0 + 15652900.0 + 6637.81, "write": 138589.0 + 79480.2 + 18300.3}
+PMC_SDF_KIB = {"fetch": 9968340.0 + 13310900.0 + 6649.31, "write": 141933.0 + 79525.6 + 18371.2}
 PMC_COPY_KIB = {"fetch": 524308.0, "write": 1048576.0}
-PMC_SDF_L2_HIT = 603713610.0 / (603713610.0 + 220420489.0)    # TCC_HIT_sum / (HIT + MISS), three launches
+PMC_SDF_L2_HIT = 531064500.0 / (531064500.0 + 233869083.0)    # TCC_HIT_sum / (HIT + MISS), three launches
 COPY_BYTES = 1 << 30
 
 
