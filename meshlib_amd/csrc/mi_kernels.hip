@@ -555,6 +555,30 @@ __device__ __forceinline__ Quad loadQuad(const MeshDev& m, int j)
     return Q;
 }
 
+// A lower-cost box distance for pruning only: the same clamped terms, summed with fused
+// multiply-adds.  Every term is non-negative, so it and boxDistSq are both within 3 units in the
+// last place of the exact sum, i.e. within 2^-21 relative of each other: a walk that prunes with
+// `slotDistFast < best * (1 + 2^-19)` (boundAbove) tests everything the exact test keeps.
+__device__ __forceinline__ float slotDistFast(const Quad& Q, int i, float px, float py, float pz)
+{
+    const float tx = __builtin_amdgcn_fmed3f(px, Q.mn[i][0], Q.mx[i][0]) - px;
+    const float ty = __builtin_amdgcn_fmed3f(py, Q.mn[i][1], Q.mx[i][1]) - py;
+    const float tz = __builtin_amdgcn_fmed3f(pz, Q.mn[i][2], Q.mx[i][2]) - pz;
+    return __builtin_fmaf(tx, tx, __builtin_fmaf(ty, ty, tz * tz));
+}
+
+// best * (1 + 2^-19): the threshold of slotDistFast tests (+inf above FLT_MAX / (1 + 2^-19); a
+// negative best, the opted-out lanes', stays negative)
+__device__ __forceinline__ float boundAbove(float best) { return best * 1.0000019073486328f; }
+
+// a value no larger than the exact box distance behind a slotDistFast result `d` (16 units in the
+// last place below it), for the stack's lower bounds
+__device__ __forceinline__ float belowFast(float d)
+{
+    const uint32_t b = __float_as_uint(d);
+    return __uint_as_float(b > 16u ? b - 16u : 0u);
+}
+
 __device__ __forceinline__ float slotDistSq(const Quad& Q, int i, float px, float py, float pz)
 {
     return boxDistSq(Q.mn[i][0], Q.mn[i][1], Q.mn[i][2], Q.mx[i][0], Q.mx[i][1], Q.mx[i][2], px, py, pz);
@@ -612,8 +636,8 @@ __device__ __forceinline__ float flushLeaves(const MeshDev& m, float px, float p
 // The walk of coopDistance from node `cur` (box distance `curd`), for lanes whose best so far is
 // `best`; after `budget` visited steps (0 = no limit) it stops and sets `stopped` (wave-uniform),
 // returning the best found so far -- a real candidate, so a later walk can finish from it.
-// A slot box lies inside its parent's box, so its distance is never below the parent's (the float
-// box distance is monotone in the box): a lane that passes a slot test passed the parent's.
+// Pruning compares slotDistFast distances with boundAbove(best), a superset of the exact tests:
+// it may add visits, never drop one.  Only the leaf tests decide the result.
 template <int D>
 __device__ __forceinline__ float coopDistanceWalk(const MeshDev& m, float px, float py, float pz, float best,
                                                   int cur, float curd, DistStack<D>* stk, int budget, bool& stopped)
@@ -624,6 +648,7 @@ __device__ __forceinline__ float coopDistanceWalk(const MeshDev& m, float px, fl
     int refs = 0;      // lane e: node ref of stack entry e
     bool have = true;
     stk->best[lane] = __float_as_uint(best);
+    float bestHi = boundAbove(best);     // every pruning test compares against this (slotDistFast)
     for (;;) {
         if (!have) {
             if (sp == 0) break;
@@ -632,7 +657,7 @@ __device__ __forceinline__ float coopDistanceWalk(const MeshDev& m, float px, fl
             curd = stackDist(stk->d[sp][lane]);
         }
         have = false;
-        uint64_t act = ltMask(curd, best);
+        uint64_t act = ltMask(curd, bestHi);
         if (!act) continue;
         if (budget && ++steps > budget) {
             stopped = true;
@@ -641,10 +666,11 @@ __device__ __forceinline__ float coopDistanceWalk(const MeshDev& m, float px, fl
         if (cur < 0) {
             if (qn + __popcll(act) > 64) {
                 best = flushLeaves<D>(m, px, py, pz, stk, qn);
+                bestHi = boundAbove(best);
                 qn = 0;
-                act = ltMask(curd, best);
+                act = ltMask(curd, bestHi);
             }
-            if (curd < best) {
+            if (curd < bestHi) {
                 const int r = __builtin_amdgcn_mbcnt_hi((uint32_t)(act >> 32),
                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)act, 0u));
                 stk->leaf[qn + r] = ~cur;
@@ -658,9 +684,11 @@ __device__ __forceinline__ float coopDistanceWalk(const MeshDev& m, float px, fl
         uint64_t b[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            d[i] = slotDistSq(Q, i, px, py, pz);
-            b[i] = ltMask(d[i], best);
+            d[i] = slotDistFast(Q, i, px, py, pz);
+            b[i] = ltMask(d[i], bestHi);
         }
+        if (Q.leafA) b[1] = 0;     // empty slots
+        if (Q.leafB) b[3] = 0;
         const uint64_t any = b[0] | b[1] | b[2] | b[3];
         if (!any) continue;
         // order keys: the lowest active lane's distance bits (non-negative floats order like their
@@ -686,7 +714,7 @@ __device__ __forceinline__ float coopDistanceWalk(const MeshDev& m, float px, fl
             } else {
                 const int pos = sp + farther;
                 refs = mi_writeLane(Q.ref[i], pos, refs);
-                stk->d[pos][lane] = stackDist(d[i]);
+                stk->d[pos][lane] = stackDist(belowFast(d[i]));
             }
         }
         sp += cnt - 1;
@@ -823,11 +851,12 @@ __device__ __forceinline__ void flushProjLeaves(const MeshDev& m, float px, floa
     bestKey = (uint32_t)b;
 }
 
-// lanes whose (d, key) is below (bd, bk) in the order (distance, key)
-__device__ __forceinline__ uint64_t lexMask(float d, uint32_t key, float bd, uint32_t bk)
-{
-    return ltMask(d, bd) | (__builtin_amdgcn_fcmpf(d, bd, 1) & __builtin_amdgcn_uicmp(key, bk, 36));
-}
+// lanes with a <= b (ordered compare)
+__device__ __forceinline__ uint64_t leMask(float a, float b) { return __builtin_amdgcn_fcmpf(a, b, 5); }
+
+// the `<=` pruning limit of a projection walk: its best, with +inf lowered to FLT_MAX so that an
+// empty quad slot (+inf box) never passes; every finite box distance still does
+__device__ __forceinline__ float pruneLimit(float best) { return best == INFINITY ? FLT_MAX : best; }
 
 // record = false leaves res's point, face and barycentrics unset (res.face >= 0 still marks an
 // improvement): for callers that keep only (distSq, key).
@@ -842,6 +871,7 @@ __device__ __forceinline__ void coopProjectWalk(const MeshDev& m, float px, floa
     int sp = 0, steps = 0, qn = 0;
     stk->best[lane] = ((unsigned long long)__float_as_uint(res.distSq) << 32) | bestKey;
     stk->win[lane] = -1;
+    float bestT = pruneLimit(res.distSq);
     // lane e of each: stack entry e's node ref, depth | 0x100 (relation: child of the other child),
     // and for that relation the 64-lane mask of the entry's own key bit
     int refs = 0, infos = 0, masksLo = 0, masksHi = 0;
@@ -868,7 +898,9 @@ __device__ __forceinline__ void coopProjectWalk(const MeshDev& m, float px, floa
         have = false;
         // popped distances are lower bounds: the test may keep a lane whose box lies just past its
         // best (its slot and leaf tests are exact), never drops one that could still win
-        uint64_t act = lexMask(curd, curKey, res.distSq, bestKey);
+        // visits keep every (distance, key) below (best, bestKey): distance <= best is a superset
+        // (distance ties with a larger key only add visits; the leaf tests order by key exactly)
+        uint64_t act = leMask(curd, bestT);
         if (!act) continue;
         if (budget && ++steps > budget) {
             stopped = true;
@@ -877,8 +909,9 @@ __device__ __forceinline__ void coopProjectWalk(const MeshDev& m, float px, floa
         if (cur < 0) {
             if (qn + __popcll(act) > 64) {
                 flushProjLeaves<D>(m, px, py, pz, stk, qn, res.distSq, bestKey);
+                bestT = pruneLimit(res.distSq);
                 qn = 0;
-                act = lexMask(curd, curKey, res.distSq, bestKey);
+                act = leMask(curd, bestT);
             }
             if ((act >> lane) & 1ull) {
                 const int r = __builtin_amdgcn_mbcnt_hi((uint32_t)(act >> 32),
@@ -913,7 +946,7 @@ __device__ __forceinline__ void coopProjectWalk(const MeshDev& m, float px, floa
         // the binary walk would have reached it
         uint64_t b[4];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) b[i] = lexMask(d[i], key[i], res.distSq, bestKey);
+        for (int i = 0; i < 4; ++i) b[i] = leMask(d[i], bestT);
         const uint64_t any = b[0] | b[1] | b[2] | b[3];
         if (!any) continue;
         // order keys as in coopDistanceWalk: lowest active lane's distance bits, slot index below
