@@ -59,6 +59,7 @@ def pmc_traffic_bytes():
 
 TRAIN_N, TRAIN_BATCH, TRAIN_LR, TRAIN_SEED = 1 << 20, 8192, 1e-3, 11
 MARGINS = (0.05, 0.1, 0.25)
+BATCH_TRAIN_N, BATCH_TRAIN_STEPS = 1 << 18, 500
 # FLOP per sample: forward 2*(99*128 + 3*128*128 + 128); backward 2 GEMMs per layer (dW, dH)
 MLP_FWD_FLOP = 2 * (99 * 128 + 3 * 128 * 128 + 128)
 MLP_BWD_FLOP = 2 * (99 * 128 + 3 * 128 * 128 + 128) + 2 * (3 * 128 * 128 + 128)
@@ -145,7 +146,7 @@ def batched_main(args):
     sides = rng.choice([32, 64, 96, 128, 192], size=n_items)
     nqs = rng.integers(1 << 18, (1 << 21) + 1, size=n_items)
     t0 = time.time()
-    items, keep, per_item = [], [], []
+    items, keep, per_item, geoms = [], [], [], []
     tot_vox = tot_pts = 0
     for i in range(n_items):
         v, f = synth.bumpy_icosphere(int(levels[i]), 32, 1000 + i)
@@ -161,6 +162,7 @@ def batched_main(args):
         pr = M.PointsToMeshProjector()
         pr.updateMeshData(mesh)
         per_item.append((fwn, pr, dims, xf, ds, dp, dr))
+        geoms.append((v, f))
         keep += [dp, ds, dr]
         tot_vox += int(np.prod(dims))
         tot_pts += int(nqs[i])
@@ -192,11 +194,47 @@ def batched_main(args):
     t_batch = min(timed(lambda: batch.run(items, opts)) for _ in range(args.steps))
     same = all(torch.equal(ds.view(torch.int32), r[0].view(torch.int32)) and torch.equal(dr, r[1])
                for (_, _, _, _, ds, _, dr), r in zip(per_item, ref))
+    learned = None
+    if not args.no_learned:
+        # one network per mesh, trained on the GPU from its own projector's distances (2^18 samples,
+        # 500 Adam steps each), then the batch with caps (pred * (1 + m))^2 and exact reruns
+        t0 = time.time()
+        nets = []
+        for i, ((fwn, pr, dims, xf, ds, dp, dr), (v, f)) in enumerate(zip(per_item, geoms)):
+            lo, hi = v.min(0).astype(np.float64), v.max(0).astype(np.float64)
+            net = M.DistanceMLP(0.5 * (lo + hi), float(0.5 * np.max(hi - lo) * 1.2), seed=TRAIN_SEED + i)
+            tp = torch.from_numpy(synth.projection_queries(v, f, BATCH_TRAIN_N, 3000 + i)).cuda()
+            trec = torch.empty(BATCH_TRAIN_N * 8, dtype=torch.int32, device="cuda")
+            pr.find_projections_device(tp.data_ptr(), BATCH_TRAIN_N, trec.data_ptr())
+            tlab = trec.view(torch.float32).view(-1, 8)[:, 7].sqrt().contiguous()
+            net.train(tp, tlab, steps=BATCH_TRAIN_STEPS, batch=TRAIN_BATCH, lr=TRAIN_LR)
+            nets.append(net)
+        torch.cuda.synchronize()
+        train_s = time.time() - t0
+        learned = {"train_s_all_meshes": train_s, "samples_per_mesh": BATCH_TRAIN_N, "steps_per_mesh": BATCH_TRAIN_STEPS,
+                   "margins": {}}
+        for m in (0.1, 0.25):
+            reruns = [0]
+
+            def go():
+                reruns[0] = batch.run_learned(items, nets, m, opts)
+            t_l = min(timed(go) for _ in range(args.steps))
+            same_l = all(torch.equal(ds.view(torch.int32), r[0].view(torch.int32)) and torch.equal(dr, r[1])
+                         for (_, _, _, _, ds, _, dr), r in zip(per_item, ref))
+            learned["margins"][str(m)] = {"ms": t_l, "reruns": reruns[0],
+                                          "rerun_rate": reruns[0] / (tot_vox + tot_pts),
+                                          "identical_to_per_item": bool(same_l),
+                                          "speedup_vs_batched_plain": t_batch / t_l,
+                                          "speedup_vs_per_item_coop": t_items / t_l}
+            log(f"learned+batched m={m}: {t_l:.1f} ms, reruns {reruns[0]}, identical {same_l}")
     t_plain = None
     if not args.no_plain:
         M.set_traversal("per_lane")
         t_plain = timed(per_item_run)
         M.set_traversal("wave_coop")
+        if learned:
+            for e in learned["margins"].values():
+                e["speedup_vs_per_item_per_lane"] = t_plain / e["ms"]
     out = {"metric": "signed-dist voxels/s & projected pts/s (batched uneven workload)",
            "value": (tot_vox + tot_pts) / (t_batch / 1e3), "unit": "queries/s (voxels + projected points)",
            "n_gpus": 1, "steps": args.steps, "ms_per_step": t_batch, "higher_is_better": True,
@@ -207,7 +245,7 @@ def batched_main(args):
            "per_item_coop_ms": t_items, "per_item_per_lane_ms": t_plain,
            "speedup_vs_per_item_coop": t_items / t_batch,
            "speedup_vs_per_item_per_lane": (t_plain / t_batch) if t_plain else None,
-           "identical_to_per_item": bool(same)}
+           "identical_to_per_item": bool(same), "learned_batched": learned}
     print(json.dumps(out), flush=True)
 
 

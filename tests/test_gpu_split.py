@@ -105,3 +105,25 @@ def test_split_projections_identical(level, bumps, nq, restore_budget):
     if level <= 4:
         o = O.Mesh(v, f).find_projections(pts)
         assert np.array_equal(ref.view(np.uint8), o.view(np.uint8))
+
+
+def test_infinite_limits(restore_budget):
+    """upDistLimitSq / maxDistSq = +inf: every pruning limit stays finite-or-infinite as in the
+    reference, for the plain walks and with every wave split (budget 1)."""
+    v, f = synth.bumpy_icosphere(4, 8, 5)
+    gm = M.Mesh(v, f)
+    om = O.Mesh(v, f)
+    pr = M.PointsToMeshProjector()
+    pr.updateMeshData(gm)
+    pts = synth.projection_queries(v, f, 20000, 13)
+    dims, xf, vox = synth.grid_for_mesh(v, (40, 36, 44))
+    fwn = M.FastWindingNumber(gm)
+    o_proj = om.find_projections(pts, up=float("inf"))
+    o_sdf = om.sdf_grid(dims, xf, O.default_opts(maxDistSq=float("inf")))
+    for budget in (0, 1, 64):
+        say("infinite limits, budget", budget)
+        M.set_split_budget(budget)
+        g = pr.findProjections(pts, upDistLimitSq=float("inf"))
+        assert np.array_equal(g.view(np.uint8), o_proj.view(np.uint8)), budget
+        s = fwn.calcFromGridWithDistances(dims, xf, M.DistanceToMeshOptions(maxDistSq=float("inf")))
+        assert np.array_equal(np.abs(s), np.abs(o_sdf)), budget
