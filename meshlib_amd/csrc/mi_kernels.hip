@@ -10,10 +10,17 @@
 //  * queries of a wave are spatially coherent: grid voxels are mapped as 4x4x4
 //    bricks per wave; free points are Morton-sorted first, so the lanes of a wave
 //    walk mostly the same nodes and their loads hit the same cache lines;
-//  * each lane keeps the reference's depth-first order exactly (nearer child first,
-//    strict '<' on distances), with the current node in registers and the pending
-//    far children on a per-lane stack in LDS laid out [depth][lane] (conflict-free
-//    when lanes sit at the same depth);
+//  * per-lane form (kSdfGrid, kProjections, ...): each lane keeps the reference's
+//    depth-first order exactly (nearer child first, strict '<' on distances), with
+//    the pending far children on a per-lane stack in LDS laid out [depth][lane];
+//  * wave-cooperative form (the default; coopDistanceWalk, coopProjectWalk,
+//    coopWinding): one walk per wave over 4-wide records with wave-uniform node refs
+//    (scalar loads), a stack whose uniform part is held in lanes of a few VGPRs and
+//    whose per-lane part (16-bit lower bounds of box distances) sits in LDS, and leaf
+//    tests queued and run one per lane; results equal the per-lane form's;
+//  * split form: waves past a step budget hand their queries to a second launch that
+//    walks the tree's top subtrees for them in parallel (results merged by atomic
+//    minimum) and a third that stores them;
 //  * blocks are remapped so that each XCD (blocks are dealt round-robin over the 8
 //    XCDs) gets one contiguous range of bricks / sorted queries, keeping each L2's
 //    working set spatially local.
@@ -221,16 +228,6 @@ __device__ __forceinline__ Leaf loadLeafLanes(const float4* __restrict__ tri, in
     L.b[0] = t0.w; L.b[1] = t1.x; L.b[2] = t1.y;
     L.c[0] = t1.z; L.c[1] = t1.w; L.c[2] = t2.x;
     L.face = __float_as_int(t2.y);
-    return L;
-}
-
-// The leaf record through scalar loads, issued before any divergent branch: the compiler would
-// otherwise sink the loads into the active lanes' branch and turn them into 64 vector loads.
-__device__ __forceinline__ Leaf loadLeafUniform(const float4* __restrict__ tri, int k)
-{
-    const Leaf L = loadLeaf(tri, k);
-    asm volatile("" ::"s"(L.a[0]), "s"(L.a[1]), "s"(L.a[2]), "s"(L.b[0]), "s"(L.b[1]), "s"(L.b[2]),
-                 "s"(L.c[0]), "s"(L.c[1]), "s"(L.c[2]));
     return L;
 }
 
