@@ -36,9 +36,13 @@ def log(*a):
 # `bench.py --steps 1 --warmup 0 --no-cpu --no-learned --no-plain --copy-calibration`
 # (profiles/sdf_grid_coop_k8_pmc.md): FETCH_SIZE and WRITE_SIZE in KiB summed over the three
 # dispatches, each scaled by bytes / counter reading of the 1 GiB device copy in the same run.
-PMC_SDF_KIB = {"fetch": 9968340.0 + 13310900.0 + 6649.31, "write": 141933.0 + 79525.6 + 18371.2}
+PMC_SDF_KIB = {"fetch": 9961430.0 + 13311100.0 + 6647.06, "write": 141921.0 + 79533.1 + 18371.0}
 PMC_COPY_KIB = {"fetch": 524308.0, "write": 1048576.0}
-PMC_SDF_L2_HIT = 531064500.0 / (531064500.0 + 233869083.0)    # TCC_HIT_sum / (HIT + MISS), three launches
+PMC_SDF_L2_HIT = 531287831.0 / (531287831.0 + 234057981.0)    # TCC_HIT_sum / (HIT + MISS), three launches
+# SQ_INSTS_VALU of the same three dispatches (wave instructions): the kernels are bound by VALU issue,
+# one wave64 instruction per SIMD every 4 cycles (1,024 SIMDs at 2.4 GHz peak clock)
+PMC_SDF_VALU = 3.50842e10 + 1.69419e10 + 1.95324e8
+VALU_ISSUE_PER_S = 1024 * 2.4e9 / 4
 COPY_BYTES = 1 << 30
 
 
@@ -505,7 +509,9 @@ def main():
                 "bytes_per_voxel": b_vox, "bytes_per_point": b_pt,
                 "algorithmic_bytes_per_launch": b_vox * nvox,
                 "hbm_gbs_measured": (traffic / (sdf_avg / 1000.0) / 1e9) if traffic else None,
-                "l2_hit_rate": PMC_SDF_L2_HIT if traffic else None}
+                "l2_hit_rate": PMC_SDF_L2_HIT if traffic else None,
+                "valu_insts_per_launch": PMC_SDF_VALU if traffic else None,
+                "valu_issue_frac": (PMC_SDF_VALU / (sdf_avg / 1000.0) / VALU_ISSUE_PER_S) if traffic else None}
     out = {
         "metric": "signed-dist voxels/s & projected pts/s; learned-bound vs plain GPU speedup",
         "value": value, "unit": "queries/s (voxels + projected points)",
