@@ -108,11 +108,14 @@ int mi_get_tree_builder(void);
 /* Step budgets of the split form of the wave-cooperative walks, process-wide (no counterpart in the
  * reference).  A 64-query wave still walking after `steps` node visits hands its queries, with their
  * best candidates so far, to a second launch that walks the tree's 64 top subtrees for them in
- * parallel; results are identical for every budget.  0 = never split.  Defaults: 3072 for the
- * signed-distance walk, 64 for projections.  The device entry points of one mesh share that mesh's
- * workspace: calls on different streams must not overlap in time. */
+ * parallel; results are identical for every budget.  0 = never split; MI_SPLIT_AUTO (the default)
+ * picks per call from the tree depth and the number of waves (trees of >= 18 levels: 3072 steps for
+ * the signed-distance walk, 64 for projections; shallower trees: 512 / 1024 and 256).  The device
+ * entry points of one mesh share that mesh's workspace: calls on different streams must not overlap
+ * in time. */
 #define MI_SPLIT_SDF 0
 #define MI_SPLIT_PROJECTIONS 1
+#define MI_SPLIT_AUTO (-1)
 int     mi_set_split_budget(int32_t walk, int32_t steps);
 int32_t mi_get_split_budget(int32_t walk);
 

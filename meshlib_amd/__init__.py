@@ -214,8 +214,9 @@ SPLIT_WALKS = {"sdf": 0, "projections": 1}
 
 
 def set_split_budget(steps: int, walk: str = "both"):
-    """Steps after which a wave hands its walk to the split pass (0 = never), for the signed-distance
-    walk ('sdf'), the projection walk ('projections') or both; results do not depend on it."""
+    """Steps after which a wave hands its walk to the split pass (0 = never, -1 = automatic, the
+    default), for the signed-distance walk ('sdf'), the projection walk ('projections') or both;
+    results do not depend on it."""
     for w in (SPLIT_WALKS if walk == "both" else [walk]):
         _check(lib().mi_set_split_budget(SPLIT_WALKS[w], int(steps)))
 

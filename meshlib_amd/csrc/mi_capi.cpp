@@ -24,8 +24,21 @@ namespace {
 thread_local std::string g_err;
 std::atomic<size_t> g_memLimit{ 0 };
 std::atomic<int> g_treeBuilder{ MI_TREE_BUILDER_DEVICE };
-std::atomic<int> g_splitBudget{ 3072 };       // signed-distance walks (dense, coherent bricks)
-std::atomic<int> g_splitBudgetProj{ 64 };     // projection walks (sparser, less coherent waves)
+std::atomic<int> g_splitBudget{ MI_SPLIT_AUTO };       // signed-distance walks
+std::atomic<int> g_splitBudgetProj{ MI_SPLIT_AUTO };   // projection walks
+
+// Automatic split budgets, from sweeps on MI355X (bumpy spheres of 20 k and 1.3 M triangles):
+// deep trees (>= 18 levels) walk long enough that late hand-over pays (signed distance 3072 steps,
+// within 1 % of the best from 64^3 to 512^3; projections 64, flat from 4 to 256); on shallow trees
+// the 64 subtree items per handed-over wave cost more than short walks, so hand over later
+// (signed distance 512 steps below 16 k waves, 1024 above; projections 256: a 20 k-triangle mesh's
+// 64^3 grid 2.7 -> 1.6 ms, 2^20 projections 12.5 -> 5.0 ms).
+int autoSdfBudget(int depth, int64_t waves)
+{
+    if (depth >= 18) return 3072;
+    return waves < 16384 ? 512 : 1024;
+}
+int autoProjBudget(int depth) { return depth >= 18 ? 64 : 256; }
 
 int fail(int code, const std::string& msg)
 {
@@ -183,12 +196,13 @@ int ensureWorkspace(mi_mesh* m, size_t bytes)
 // scratch of the split form for grids of up to `layers` z-layers, in `buf` (grown if needed)
 int splitScratch(mi_mesh* m, DevBuf& buf, const int32_t dims[3], int32_t layers, mi::SplitScratch& sc)
 {
-    (void)m;
     sc = mi::SplitScratch{};
-    const int budget = g_splitBudget.load();
-    if (budget <= 0 || layers <= 0) return MI_OK;
+    if (layers <= 0) return MI_OK;
     const int32_t d[3] = { dims[0], dims[1], layers };
     const int64_t waves = mi::sdfGridWaves(d, 0, layers);
+    const int setting = g_splitBudget.load();
+    const int budget = setting >= 0 ? setting : autoSdfBudget(m->depth, waves);
+    if (budget <= 0) return MI_OK;
     const size_t bytes = mi::splitScratchBytes(waves, 4);
     if (buf.bytes < bytes) MI_HIP(buf.alloc(bytes));
     mi::splitScratchBind(buf.p, waves, 4, budget, sc);
@@ -197,15 +211,21 @@ int splitScratch(mi_mesh* m, DevBuf& buf, const int32_t dims[3], int32_t layers,
 
 // split scratch of a projection launch over n points, placed at `base` (caller-sized with
 // projSplitBytes); no split when the budget is 0
-size_t projSplitBytes(int64_t n)
+int projBudget(const mi_mesh* m)
 {
-    return g_splitBudgetProj.load() > 0 ? mi::splitScratchBytes(mi::projectionWaves(n), 8) : 0;
+    const int setting = g_splitBudgetProj.load();
+    return setting >= 0 ? setting : autoProjBudget(m->depth);
 }
 
-void projSplit(void* base, int64_t n, mi::SplitScratch& sc)
+size_t projSplitBytes(const mi_mesh* m, int64_t n)
+{
+    return projBudget(m) > 0 ? mi::splitScratchBytes(mi::projectionWaves(n), 8) : 0;
+}
+
+void projSplit(const mi_mesh* m, void* base, int64_t n, mi::SplitScratch& sc)
 {
     sc = mi::SplitScratch{};
-    const int budget = g_splitBudgetProj.load();
+    const int budget = projBudget(m);
     if (budget <= 0 || !base) return;
     mi::splitScratchBind(base, mi::projectionWaves(n), 8, budget, sc);
 }
@@ -263,7 +283,7 @@ int mi_get_tree_builder(void) { return g_treeBuilder.load(); }
 
 int mi_set_split_budget(int32_t walk, int32_t steps)
 {
-    if (steps < 0) return fail(MI_E_INVALID, "negative split budget");
+    if (steps < MI_SPLIT_AUTO) return fail(MI_E_INVALID, "negative split budget");
     if (walk == MI_SPLIT_SDF) g_splitBudget.store(steps);
     else if (walk == MI_SPLIT_PROJECTIONS) g_splitBudgetProj.store(steps);
     else return fail(MI_E_INVALID, "unknown walk kind");
@@ -546,9 +566,9 @@ int mi_proj_find_projections_device(mi_mesh* m, const float* d_pts, int64_t n, c
     MI_HIP(hipSetDevice(m->device));
     hipStream_t s = pick(m, stream);
     const size_t sortBytes = align256(mi::sortWorkspaceBytes(n));
-    if (int rc = ensureWorkspace(m, sortBytes + projSplitBytes(n))) return rc;
+    if (int rc = ensureWorkspace(m, sortBytes + projSplitBytes(m, n))) return rc;
     mi::SplitScratch sc;
-    projSplit(m->ws.as<char>() + sortBytes, n, sc);
+    projSplit(m, m->ws.as<char>() + sortBytes, n, sc);
     const uint32_t* perm = nullptr;
     MI_HIP(mi::sortPointsMorton(d_pts, n, m->sortLo, m->sortHi, m->ws.p, m->ws.bytes, &perm, s));
     mi::Xf px, tx;
@@ -570,9 +590,9 @@ int mi_proj_find_projections_bounded_device(mi_mesh* m, const float* d_pts, int6
     MI_HIP(hipSetDevice(m->device));
     hipStream_t s = pick(m, stream);
     const size_t sortBytes = align256(mi::sortWorkspaceBytes(n));
-    if (int rc = ensureWorkspace(m, sortBytes + projSplitBytes(n))) return rc;
+    if (int rc = ensureWorkspace(m, sortBytes + projSplitBytes(m, n))) return rc;
     mi::SplitScratch sc;
-    projSplit(m->ws.as<char>() + sortBytes, n, sc);
+    projSplit(m, m->ws.as<char>() + sortBytes, n, sc);
     const uint32_t* perm = nullptr;
     MI_HIP(mi::sortPointsMorton(d_pts, n, m->sortLo, m->sortHi, m->ws.p, m->ws.bytes, &perm, s));
     mi::Xf px, tx;
@@ -844,9 +864,9 @@ static int pointsHost(mi_mesh* m, const float* pts, int64_t n, size_t outElem, v
     MI_HIP(dres.alloc(chunk * outElem));
     MI_HIP(ws.alloc(mi::sortWorkspaceBytes((int64_t)chunk)));
     mi::SplitScratch sc;
-    if (split && projSplitBytes((int64_t)chunk)) {
-        MI_HIP(splitBuf.alloc(projSplitBytes((int64_t)chunk)));
-        projSplit(splitBuf.p, (int64_t)chunk, sc);
+    if (split && projSplitBytes(m, (int64_t)chunk)) {
+        MI_HIP(splitBuf.alloc(projSplitBytes(m, (int64_t)chunk)));
+        projSplit(m, splitBuf.p, (int64_t)chunk, sc);
     }
     if (!report(cb, user, 0.6f)) return fail(MI_E_CANCELED, "Operation was canceled");
     const size_t chunks = ((size_t)n + chunk - 1) / chunk;

@@ -69,3 +69,15 @@ def test_tree_builder_setting():
     M.set_tree_builder("device")
     assert M.lib().mi_set_tree_builder(7) == M.MI_E_INVALID
     assert M.get_tree_builder() == "device"
+
+
+def test_split_budget_setting():
+    # automatic by default; explicit steps, 0 (off) and automatic again are accepted per walk
+    assert M.get_split_budget("sdf") == -1 and M.get_split_budget("projections") == -1
+    M.set_split_budget(4096, "sdf")
+    M.set_split_budget(0, "projections")
+    assert M.get_split_budget("sdf") == 4096 and M.get_split_budget("projections") == 0
+    assert M.lib().mi_set_split_budget(0, -2) == M.MI_E_INVALID
+    assert M.lib().mi_set_split_budget(5, 16) == M.MI_E_INVALID
+    M.set_split_budget(-1)
+    assert M.get_split_budget("sdf") == -1 and M.get_split_budget("projections") == -1

@@ -1,6 +1,6 @@
 """Split form of the signed-distance walk: waves that run past the step budget hand their voxels to a
 second launch that walks the tree's top subtrees in parallel.  The volume must not depend on the
-budget: budget 1 sends almost every wave through the split path, 0 none."""
+budget: budget 1 sends almost every wave through the split path, 0 none, -1 picks automatically."""
 import sys
 
 import numpy as np
@@ -38,7 +38,7 @@ def test_split_budgets_identical(level, bumps, grid, restore_budget):
     ref = fwn.calcFromGridWithDistances(dims, xf, opts)
     n = ref.size
     d = torch.empty(n, dtype=torch.float32, device="cuda")
-    for budget in (1, 16, 4096):
+    for budget in (1, 16, 4096, -1):
         say(f"k={level} grid {grid} budget {budget}")
         M.set_split_budget(budget)
         got = fwn.calcFromGridWithDistances(dims, xf, opts)
@@ -89,7 +89,7 @@ def test_split_projections_identical(level, bumps, nq, restore_budget):
     pr.find_projections_bounded_device(dp.data_ptr(), n, bound.data_ptr(), dr.data_ptr())
     torch.cuda.synchronize()
     ref_b = dr.cpu().numpy().copy()
-    for budget in (1, 16, 4096):
+    for budget in (1, 16, 4096, -1):
         say(f"k={level} projections budget {budget}")
         M.set_split_budget(budget)
         got = pr.findProjections(pts)
