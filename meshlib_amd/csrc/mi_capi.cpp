@@ -27,18 +27,24 @@ std::atomic<int> g_treeBuilder{ MI_TREE_BUILDER_DEVICE };
 std::atomic<int> g_splitBudget{ MI_SPLIT_AUTO };       // signed-distance walks
 std::atomic<int> g_splitBudgetProj{ MI_SPLIT_AUTO };   // projection walks
 
-// Automatic split budgets, from sweeps on MI355X (bumpy spheres of 20 k and 1.3 M triangles):
-// deep trees (>= 18 levels) walk long enough that late hand-over pays (signed distance 3072 steps,
-// within 1 % of the best from 64^3 to 512^3; projections 64, flat from 4 to 256); on shallow trees
-// the 64 subtree items per handed-over wave cost more than short walks, so hand over later
-// (signed distance 512 steps below 16 k waves, 1024 above; projections 256: a 20 k-triangle mesh's
-// 64^3 grid 2.7 -> 1.6 ms, 2^20 projections 12.5 -> 5.0 ms).
+// Automatic split budgets, from sweeps on MI355X (bumpy spheres of 20 k to 5.2 M triangles, tree
+// depths 15 to 23).  Signed distance: trees of >= 18 levels hand over after 3072 steps (within a few
+// % of the best from 64^3 to 512^3); shallower trees walk less, so the 64 subtree items of a
+// hand-over weigh more and an earlier hand-over spreads the few long waves (512 steps below 16 k
+// waves, 1024 above; 20 k triangles at 64^3: 2.7 -> 1.6 ms).  Projections: on trees of >= 20
+// levels nearly every wave is long and 64 steps is best (flat to within noise up to 2048 at 2^22
+// points); on shallower trees most waves finish alone given 1-2 k steps (2^22 points on 20 k /
+// 82 k / 328 k triangles: 36.7 / 50.8 / 54.1 ms at 64 steps, 5.0 / 10.0 / 24.0 ms at 1024-2048).
 int autoSdfBudget(int depth, int64_t waves)
 {
     if (depth >= 18) return 3072;
     return waves < 16384 ? 512 : 1024;
 }
-int autoProjBudget(int depth) { return depth >= 18 ? 64 : 256; }
+int autoProjBudget(int depth)
+{
+    if (depth >= 20) return 64;
+    return depth >= 17 ? 2048 : 1024;
+}
 
 int fail(int code, const std::string& msg)
 {
