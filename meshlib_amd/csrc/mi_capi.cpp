@@ -969,7 +969,8 @@ struct mi_batch {
     DevBuf tickets;      // 8 x u64
     DevBuf sortWs;       // per-item Morton sort workspaces
     DevBuf rerunWs;      // capped form: per-item rerun counters and lists
-    std::vector<mi::BatchItem> host;
+    std::vector<mi::BatchItem> host;     // launch order (see prepareBatch)
+    std::vector<int32_t> slotOf;          // caller's item index -> position in `host`
     std::mutex mu;
 };
 
@@ -1005,11 +1006,27 @@ int prepareBatch(mi_batch* b, const mi_batch_item* items, int32_t count, hipStre
         *stackD = std::max(*stackD, it.mesh->stackD);
     }
     if (wsTotal && b->sortWs.bytes < wsTotal) MI_HIP(b->sortWs.alloc(wsTotal));
+    // launch order: items on deeper trees first (their work units are the long ones), larger first
+    // among equals, so that the persistent launch ends on short units instead of waiting for a
+    // long one taken last
+    std::vector<int32_t> order((size_t)count);
+    for (int32_t i = 0; i < count; ++i) order[(size_t)i] = i;
+    auto units = [&](int32_t i) {
+        const mi_batch_item& it = items[i];
+        return (int64_t)it.dims[0] * it.dims[1] * it.dims[2] / 64 + it.numPts / 64;
+    };
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t c) {
+        if (items[a].mesh->depth != items[c].mesh->depth) return items[a].mesh->depth > items[c].mesh->depth;
+        return units(a) > units(c);
+    });
+    b->slotOf.assign((size_t)count, 0);
+    for (int32_t k = 0; k < count; ++k) b->slotOf[(size_t)order[(size_t)k]] = k;
     size_t off = 0;
     int64_t unit = 0;
-    for (int32_t i = 0; i < count; ++i) {
+    for (int32_t k = 0; k < count; ++k) {
+        const int32_t i = order[(size_t)k];
         const mi_batch_item& it = items[i];
-        mi::BatchItem& d = b->host[i];
+        mi::BatchItem& d = b->host[(size_t)k];
         d.m = it.mesh->dev;
         d.dx = it.dims[0]; d.dy = it.dims[1]; d.dz = it.dims[2];
         d.nbx = (d.dx + 3) / 4; d.nby = (d.dy + 3) / 4; d.nbz = (d.dz + 3) / 4;
@@ -1132,7 +1149,8 @@ int mi_batch_run_capped_device(mi_batch* b, const mi_batch_item* items, int32_t 
     size_t off = cntBytes;
     for (int32_t i = 0; i < count; ++i) {
         const mi_batch_item& it = items[i];
-        mi::BatchItem& d = b->host[i];
+        const int32_t k = b->slotOf[(size_t)i];
+        mi::BatchItem& d = b->host[(size_t)k];
         const size_t nvox = (size_t)it.dims[0] * it.dims[1] * it.dims[2];
         d.capSdf = d_capSdf ? d_capSdf[i] : nullptr;
         d.capProj = d_capProj ? d_capProj[i] : nullptr;
@@ -1140,7 +1158,7 @@ int mi_batch_run_capped_device(mi_batch* b, const mi_batch_item* items, int32_t 
         off += align256(nvox * 4);
         d.rerunPts = reinterpret_cast<uint32_t*>(base + off);
         off += align256((size_t)it.numPts * 4);
-        d.rerunCount = counts + 2 * i;
+        d.rerunCount = counts + 2 * k;
     }
     MI_HIP(mi::fillU32(counts, 2 * count, 0u, s));
     if (int rc = uploadItems(b, count, s)) return rc;
@@ -1150,9 +1168,9 @@ int mi_batch_run_capped_device(mi_batch* b, const mi_batch_item* items, int32_t 
     MI_HIP(hipMemcpyAsync(hc.data(), counts, hc.size() * 4, hipMemcpyDeviceToHost, s));
     MI_HIP(hipStreamSynchronize(s));
     int64_t unit = 0, again = 0;
-    for (int32_t i = 0; i < count; ++i) {
-        mi::BatchItem& d = b->host[i];
-        const uint32_t nv = hc[2 * i], np = hc[2 * i + 1];
+    for (int32_t k = 0; k < count; ++k) {     // launch order, as the counters
+        mi::BatchItem& d = b->host[(size_t)k];
+        const uint32_t nv = hc[2 * (size_t)k], np = hc[2 * (size_t)k + 1];
         again += (int64_t)nv + np;
         d.voxList = d.rerunVox;
         d.nVoxList = nv;
