@@ -77,6 +77,24 @@ __device__ __forceinline__ int logicalBlock(int nLogical)
     return (b % kXcds) * per + b / kXcds;
 }
 
+// Grid kernels instead deal chunks of kGridChunk consecutive logical blocks (64 bricks) to the XCDs
+// in turn: each L2 still sees spatially compact runs, and the costly middle of a grid (voxels deep
+// inside the mesh) is spread over all eight XCDs instead of landing on two of them.  The launch
+// covers whole rounds of chunks (chunkedGrid), so the mapping is a bijection onto its blocks.
+constexpr int kGridChunk = 16;
+__device__ __forceinline__ int gridBlock()
+{
+    const int b = blockIdx.x;
+    const int x = b % kXcds, r = b / kXcds;
+    return ((r / kGridChunk) * kXcds + x) * kGridChunk + r % kGridChunk;
+}
+
+inline int chunkedGrid(int nLogical)
+{
+    const int round = kXcds * kGridChunk;
+    return (nLogical + round - 1) / round * round;
+}
+
 // blocks of a persistent launch: enough resident waves to fill every CU
 inline int persistentBlocks()
 {
@@ -384,7 +402,7 @@ __global__ __launch_bounds__(kBlock) void kSdfGrid(MeshDev m, int dx, int dy, in
                                                    DistOpts o, float* __restrict__ out)
 {
     __shared__ int2 smem[D * kBlock];
-    const int lb = logicalBlock(nLogical);
+    const int lb = gridBlock();
     if (lb >= nLogical) return;
     const int lane = threadIdx.x & 63;
     const int brick = lb * (kBlock / 64) + (threadIdx.x >> 6);
@@ -413,7 +431,7 @@ __global__ __launch_bounds__(kBlock) void kFwnGrid(MeshDev m, int dx, int dy, in
                                                    float beta, float* __restrict__ out)
 {
     __shared__ int2 smem[D * kBlock];
-    const int lb = logicalBlock(nLogical);
+    const int lb = gridBlock();
     if (lb >= nLogical) return;
     const int lane = threadIdx.x & 63;
     const int brick = lb * (kBlock / 64) + (threadIdx.x >> 6);
@@ -1288,7 +1306,7 @@ __global__ __launch_bounds__(kBlock) void kSdfGridCoop(MeshDev m, int dx, int dy
                                                        SplitScratch sc, float* __restrict__ out)
 {
     __shared__ DistStack<D> stkMem[kWaves];
-    const int lb = logicalBlock(nLogical);
+    const int lb = gridBlock();
     if (lb >= nLogical) return;
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
@@ -1409,7 +1427,7 @@ __global__ __launch_bounds__(kBlock) void kFwnGridCoop(MeshDev m, int dx, int dy
                                                        float beta, float* __restrict__ out)
 {
     __shared__ uint4 stkMem[kWaves][D];
-    const int lb = logicalBlock(nLogical);
+    const int lb = gridBlock();
     if (lb >= nLogical) return;
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
@@ -1780,7 +1798,7 @@ hipError_t launchSdfGrid(const MeshDev& m, const int32_t dims[3], const Xf& xf, 
     return dispatchDepth(stackD, [&](auto Dc) {
         constexpr int D = decltype(Dc)::value;
         if (coop) {
-            hipLaunchKernelGGL(kSdfGridCoop<D>, dim3(paddedGrid(nLogical)), dim3(kBlock), 0, s, m, dims[0],
+            hipLaunchKernelGGL(kSdfGridCoop<D>, dim3(chunkedGrid(nLogical)), dim3(kBlock), 0, s, m, dims[0],
                                dims[1], zBegin, zEnd, nbx, nby, nbz, nLogical, xf, o, bound, rerun, rerunCount,
                                sc, out);
             if (sc.list) {
@@ -1791,7 +1809,7 @@ hipError_t launchSdfGrid(const MeshDev& m, const int32_t dims[3], const Xf& xf, 
                                    zBegin, zEnd, nbx, nby, nbz, xf, o, sc, out);
             }
         } else {
-            hipLaunchKernelGGL(kSdfGrid<D>, dim3(paddedGrid(nLogical)), dim3(kBlock), 0, s, m, dims[0],
+            hipLaunchKernelGGL(kSdfGrid<D>, dim3(chunkedGrid(nLogical)), dim3(kBlock), 0, s, m, dims[0],
                                dims[1], zBegin, zEnd, nbx, nby, nbz, nLogical, xf, o, out);
         }
         return hipGetLastError();
@@ -1824,10 +1842,10 @@ hipError_t launchFwnGrid(const MeshDev& m, const int32_t dims[3], const Xf& xf, 
     return dispatchDepth(stackD, [&](auto Dc) {
         constexpr int D = decltype(Dc)::value;
         if (coop)
-            hipLaunchKernelGGL(kFwnGridCoop<D>, dim3(paddedGrid(nLogical)), dim3(kBlock), 0, s, m, dims[0],
+            hipLaunchKernelGGL(kFwnGridCoop<D>, dim3(chunkedGrid(nLogical)), dim3(kBlock), 0, s, m, dims[0],
                                dims[1], zBegin, zEnd, nbx, nby, nbz, nLogical, xf, beta, out);
         else
-            hipLaunchKernelGGL(kFwnGrid<D>, dim3(paddedGrid(nLogical)), dim3(kBlock), 0, s, m, dims[0],
+            hipLaunchKernelGGL(kFwnGrid<D>, dim3(chunkedGrid(nLogical)), dim3(kBlock), 0, s, m, dims[0],
                                dims[1], zBegin, zEnd, nbx, nby, nbz, nLogical, xf, beta, out);
         return hipGetLastError();
     });
