@@ -36,9 +36,9 @@ def log(*a):
 # `bench.py --steps 1 --warmup 0 --no-cpu --no-learned --no-plain --copy-calibration`
 # (profiles/sdf_grid_coop_k8_pmc.md): FETCH_SIZE and WRITE_SIZE in KiB summed over the three
 # dispatches, each scaled by bytes / counter reading of the 1 GiB device copy in the same run.
-PMC_SDF_KIB = {"fetch": 9961430.0 + 13311100.0 + 6647.06, "write": 141921.0 + 79533.1 + 18371.0}
+PMC_SDF_KIB = {"fetch": 21609800.0 + 11811100.0 + 6525.25, "write": 135692.0 + 79533.1 + 18294.6}
 PMC_COPY_KIB = {"fetch": 524308.0, "write": 1048576.0}
-PMC_SDF_L2_HIT = 531287831.0 / (531287831.0 + 234057981.0)    # TCC_HIT_sum / (HIT + MISS), three launches
+PMC_SDF_L2_HIT = 438451700.0 / (438451700.0 + 334205131.0)    # TCC_HIT_sum / (HIT + MISS), three launches
 # SQ_INSTS_VALU of the same three dispatches (wave instructions): the kernels are bound by VALU issue,
 # one wave64 instruction per SIMD every 4 cycles (1,024 SIMDs at 2.4 GHz peak clock)
 PMC_SDF_VALU = 3.50842e10 + 1.69419e10 + 1.95324e8
