@@ -306,15 +306,15 @@ int32_t mi_get_split_budget(int32_t walk)
 namespace {
 
 struct ArenaLayout {
-    size_t dnode, tri, dint, dleaf, dquad, cut;
-    size_t total() const { return dnode + tri + dint + dleaf + dquad + cut; }
+    size_t dnode, tri, dint, dleaf, dquad, cut, prism;
+    size_t total() const { return dnode + tri + dint + dleaf + dquad + cut + prism; }
 };
 
 ArenaLayout arenaLayout(int32_t nf)
 {
     const size_t ni = nf > 0 ? (size_t)nf - 1 : 0, nl = nf > 0 ? (size_t)nf : 0;
     return { align256(ni * 64), align256(nl * 48), align256(ni * 32), align256(nl * 32), align256(ni * 128),
-             nf > 0 ? align256(mi::kMaxCut * sizeof(mi::CutEntry)) + 256 : 0 };
+             nf > 0 ? align256(mi::kMaxCut * sizeof(mi::CutEntry)) + 256 : 0, align256(nl * 80) };
 }
 
 void bindArena(mi_mesh* m, const ArenaLayout& L)
@@ -325,6 +325,7 @@ void bindArena(mi_mesh* m, const ArenaLayout& L)
     m->dev.dint = reinterpret_cast<const float4*>(base + L.dnode + L.tri);
     m->dev.dleaf = reinterpret_cast<const float4*>(base + L.dnode + L.tri + L.dint);
     m->dev.dquad = reinterpret_cast<const float4*>(base + L.dnode + L.tri + L.dint + L.dleaf);
+    m->dev.prism = reinterpret_cast<const float4*>(base + L.dnode + L.tri + L.dint + L.dleaf + L.dquad + L.cut);
 }
 
 // 4-wide records of the cooperative walks and the cut of the split form, from the binary records
@@ -336,6 +337,9 @@ int finishArena(mi_mesh* m, const ArenaLayout& L)
     if (m->nf >= 2)
         MI_HIP(mi::buildQuads(m->dev.dnode, m->nf - 1,
                               reinterpret_cast<float4*>(base + L.dnode + L.tri + L.dint + L.dleaf), m->stream));
+    MI_HIP(mi::buildPrisms(m->dev.tri, m->nf,
+                           reinterpret_cast<float4*>(base + L.dnode + L.tri + L.dint + L.dleaf + L.dquad + L.cut),
+                           m->stream));
     char* cutBase = base + L.dnode + L.tri + L.dint + L.dleaf + L.dquad;
     mi::CutEntry* cut = reinterpret_cast<mi::CutEntry*>(cutBase);
     int32_t* nCutDev = reinterpret_cast<int32_t*>(cutBase + align256(mi::kMaxCut * sizeof(mi::CutEntry)));

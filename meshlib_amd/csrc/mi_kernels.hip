@@ -602,6 +602,33 @@ __device__ __forceinline__ float slotDistSq(const Quad& Q, int i, float px, floa
 // lanes with a < b (ordered compare) as a wave mask, straight from the compare
 __device__ __forceinline__ uint64_t ltMask(float a, float b) { return __builtin_amdgcn_fcmpf(a, b, 4); }
 
+// A lower bound of the squared distance from (px, py, pz) to the triangle of leaf slot `slot`, at
+// most the leaf test's own result (the FP64 closest point, rounded).  The slot's record holds a unit
+// normal n with the range [lo, hi] of n.x over the triangle, and for each edge an in-plane unit
+// normal m_e with the maximum of m_e.x over the triangle: the triangle lies in the slab and in
+// each half-space, so its distance is at least sqrt(a^2 + b^2), a = distance to the slab, b = the
+// largest distance to an edge half-space (n and m_e are orthogonal).  Rounding is covered by
+// eps = 2^-20 (|p|_1 + r), r = max |v|_1 over the triangle's vertices (16 units of 2^-24 on every
+// magnitude involved, here and in the leaf test's rounding to float), and by the factor 1 - 2^-13
+// (the float normals' length and orthogonality, the sums of squares).  A degenerate or non-finite
+// triangle has an all-zero record and the bound 0.
+// The record is read with scalar loads: `slot` is wave-uniform.
+__device__ __forceinline__ float leafLowerBound(const MeshDev& m, int slot, float px, float py, float pz)
+{
+    const float4 r0 = cload(m.prism, 5 * (size_t)slot), r1 = cload(m.prism, 5 * (size_t)slot + 1);
+    const float4 r2 = cload(m.prism, 5 * (size_t)slot + 2), r3 = cload(m.prism, 5 * (size_t)slot + 3);
+    const float4 r4 = cload(m.prism, 5 * (size_t)slot + 4);
+    const float np = __builtin_fmaf(r0.z, pz, __builtin_fmaf(r0.y, py, r0.x * px));
+    const float a = fmaxf(np - r1.x, r0.w - np);
+    const float e0 = __builtin_fmaf(r1.w, pz, __builtin_fmaf(r1.z, py, __builtin_fmaf(r1.y, px, -r2.x)));
+    const float e1 = __builtin_fmaf(r2.w, pz, __builtin_fmaf(r2.z, py, __builtin_fmaf(r2.y, px, -r3.x)));
+    const float e2 = __builtin_fmaf(r3.w, pz, __builtin_fmaf(r3.z, py, __builtin_fmaf(r3.y, px, -r4.x)));
+    const float b = fmaxf(e0, fmaxf(e1, e2));
+    const float eps = (fabsf(px) + fabsf(py) + fabsf(pz) + r4.y) * 0x1p-20f;
+    const float as = fmaxf(a - eps, 0.f), bs = fmaxf(b - eps, 0.f);
+    return __builtin_fmaf(as, as, bs * bs) * (1.f - 0x1p-13f);
+}
+
 // Stack of a wave-cooperative distance walk.  The node ref of entry e is held by lane e of one
 // VGPR (writelane / readlane move wave-uniform values without memory), and each lane's box
 // distance of the entry sits in LDS at d[e][lane]: a pop reads one value per lane instead of
@@ -679,13 +706,16 @@ __device__ __forceinline__ float coopDistanceWalk(const MeshDev& m, float px, fl
             break;
         }
         if (cur < 0) {
+            // lanes whose triangle may still lie closer than their best (the leaf test decides)
+            const uint64_t may = ~(uint64_t)__builtin_amdgcn_fcmpf(leafLowerBound(m, ~cur, px, py, pz), bestHi, 2);
+            act &= may;
             if (qn + __popcll(act) > 64) {
                 best = flushLeaves<D>(m, px, py, pz, stk, qn);
                 bestHi = boundAbove(best);
                 qn = 0;
-                act = ltMask(curd, bestHi);
+                act = ltMask(curd, bestHi) & may;
             }
-            if (curd < bestHi) {
+            if ((act >> lane) & 1ull) {
                 const int r = __builtin_amdgcn_mbcnt_hi((uint32_t)(act >> 32),
                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)act, 0u));
                 stk->leaf[qn + r] = ~cur;
@@ -1681,6 +1711,68 @@ __global__ void kBuildCut(const float4* __restrict__ dnode, CutEntry root, CutEn
     *nCut = n;
 }
 
+__device__ float roundUpF(double x)
+{
+    float f = (float)x;
+    if ((double)f < x) f = nextafterf(f, INFINITY);
+    return f;
+}
+__device__ float roundDownF(double x)
+{
+    float f = (float)x;
+    if ((double)f > x) f = nextafterf(f, -INFINITY);
+    return f;
+}
+
+// One leaf slot's plane record (leafMayWin): normal and edge normals in FP64, rounded to float,
+// then the offsets of those float planes over the three vertices in FP64 (products of floats are
+// exact there), widened by r * 2^-40 and rounded outwards.
+__global__ void kBuildPrisms(const float4* __restrict__ tri, int n, float4* __restrict__ prism)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 t0 = tri[3 * (size_t)i], t1 = tri[3 * (size_t)i + 1], t2 = tri[3 * (size_t)i + 2];
+    const double v[3][3] = { { t0.x, t0.y, t0.z }, { t0.w, t1.x, t1.y }, { t1.z, t1.w, t2.x } };
+    double r = 0;
+    for (int j = 0; j < 3; ++j) r = fmax(r, fabs(v[j][0]) + fabs(v[j][1]) + fabs(v[j][2]));
+    float4* out = prism + 5 * (size_t)i;
+    const double e0[3] = { v[1][0] - v[0][0], v[1][1] - v[0][1], v[1][2] - v[0][2] };
+    const double e1[3] = { v[2][0] - v[0][0], v[2][1] - v[0][1], v[2][2] - v[0][2] };
+    const double nn[3] = { e0[1] * e1[2] - e0[2] * e1[1], e0[2] * e1[0] - e0[0] * e1[2], e0[0] * e1[1] - e0[1] * e1[0] };
+    const double len = sqrt(nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2]);
+    if (!(r < INFINITY) || !(len > 0) || !(len < INFINITY)) {
+        for (int k = 0; k < 5; ++k) out[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+        return;
+    }
+    const double nd[3] = { nn[0] / len, nn[1] / len, nn[2] / len };
+    float pl[4][3];     // normal, then the three edge normals
+    for (int k = 0; k < 3; ++k) pl[0][k] = (float)nd[k];
+    for (int e = 0; e < 3; ++e) {
+        const double* p0 = v[e];
+        const double* p1 = v[(e + 1) % 3];
+        const double d[3] = { p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2] };
+        double mm[3] = { d[1] * nd[2] - d[2] * nd[1], d[2] * nd[0] - d[0] * nd[2], d[0] * nd[1] - d[1] * nd[0] };
+        const double t = mm[0] * nd[0] + mm[1] * nd[1] + mm[2] * nd[2];
+        for (int k = 0; k < 3; ++k) mm[k] -= t * nd[k];
+        const double ml = sqrt(mm[0] * mm[0] + mm[1] * mm[1] + mm[2] * mm[2]);
+        for (int k = 0; k < 3; ++k) pl[1 + e][k] = ml > 0 ? (float)(mm[k] / ml) : 0.f;
+    }
+    double hi[4], lo = INFINITY;
+    for (int q = 0; q < 4; ++q) hi[q] = -INFINITY;
+    for (int j = 0; j < 3; ++j)
+        for (int q = 0; q < 4; ++q) {
+            const double s = (double)pl[q][0] * v[j][0] + (double)pl[q][1] * v[j][1] + (double)pl[q][2] * v[j][2];
+            hi[q] = fmax(hi[q], s);
+            if (q == 0) lo = fmin(lo, s);
+        }
+    const double marg = r * 0x1p-40;
+    out[0] = make_float4(pl[0][0], pl[0][1], pl[0][2], roundDownF(lo - marg));
+    out[1] = make_float4(roundUpF(hi[0] + marg), pl[1][0], pl[1][1], pl[1][2]);
+    out[2] = make_float4(roundUpF(hi[1] + marg), pl[2][0], pl[2][1], pl[2][2]);
+    out[3] = make_float4(roundUpF(hi[2] + marg), pl[3][0], pl[3][1], pl[3][2]);
+    out[4] = make_float4(roundUpF(hi[3] + marg), roundUpF(r), 0.f, 0.f);
+}
+
 __global__ void kFillU32(uint32_t* p, int n, uint32_t v)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1734,6 +1826,13 @@ hipError_t buildCut(const float4* dnode, int32_t rootRef, const float rootMin[3]
     root.ref = rootRef;
     for (int i = 0; i < 3; ++i) { root.mn[i] = rootMin[i]; root.mx[i] = rootMax[i]; }
     hipLaunchKernelGGL(kBuildCut, dim3(1), dim3(64), 0, s, dnode, root, cut, nCut);
+    return hipGetLastError();
+}
+
+hipError_t buildPrisms(const float4* tri, int32_t numLeaves, float4* prism, hipStream_t s)
+{
+    if (numLeaves <= 0) return hipSuccess;
+    hipLaunchKernelGGL(kBuildPrisms, dim3((numLeaves + 255) / 256), dim3(256), 0, s, tri, numLeaves, prism);
     return hipGetLastError();
 }
 

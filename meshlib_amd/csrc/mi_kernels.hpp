@@ -25,6 +25,7 @@ struct MeshDev {
     const float4* dleaf;   // 2 per leaf slot
     const float4* dquad;   // 8 per internal node: its grandchildren (4-wide records of the cooperative walks)
     const CutEntry* cut;   // nCut subtree roots covering every leaf once
+    const float4* prism;   // 5 per leaf slot: planes bounding the triangle (leaf filter of the walks)
     int32_t nCut;
     float rootMin[3], rootMax[3];
     int32_t rootRef;
@@ -83,6 +84,8 @@ int64_t projectionWaves(int64_t n);
 // 4-wide records from the binary records: internal node j's four grandchildren, a leaf child
 // standing for itself in one slot with an empty partner slot
 hipError_t buildQuads(const float4* dnode, int32_t numInternal, float4* dquad, hipStream_t s);
+// the leaf-slot plane records of MeshDev::prism from the triangle records
+hipError_t buildPrisms(const float4* tri, int32_t numLeaves, float4* prism, hipStream_t s);
 
 // the cut of the tree at depth kCutDepth into at most kMaxCut subtrees (cut[], *nCut on the device)
 hipError_t buildCut(const float4* dnode, int32_t rootRef, const float rootMin[3], const float rootMax[3],
