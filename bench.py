@@ -36,14 +36,14 @@ def log(*a):
 # `bench.py --steps 1 --warmup 0 --no-cpu --no-learned --no-plain --copy-calibration`
 # (profiles/sdf_grid_coop_k8_pmc.md): FETCH_SIZE and WRITE_SIZE in KiB summed over the three
 # dispatches, each scaled by bytes / counter reading of the 1 GiB device copy in the same run.
-PMC_SDF_KIB = {"fetch": 36025600.0 + 18741300.0 + 6647.44, "write": 135013.0 + 79588.6 + 18328.0}
+PMC_SDF_KIB = {"fetch": 41281300.0 + 9248390.0 + 2261.06, "write": 139609.0 + 22481.8 + 5016.75}
 PMC_COPY_KIB = {"fetch": 524308.0, "write": 1048576.0}
-PMC_SDF_L2_HIT = 721031300.0 / (721031300.0 + 487221750.0)    # TCC_HIT_sum / (HIT + MISS), three launches
+PMC_SDF_L2_HIT = 651274420.0 / (651274420.0 + 448592343.0)    # TCC_HIT_sum / (HIT + MISS), three launches
 # SQ_INSTS_VALU of the same three dispatches (wave instructions), against the VALU issue rate of one
 # wave64 instruction per SIMD every 4 cycles (1,024 SIMDs at 2.4 GHz peak clock).  The walks are bound
 # by the latency of their dependent scalar loads rather than by issue: cutting kSdfGridCoop's VALU
 # instructions by 23 % shortened it by 2 %, a seventh wave per SIMD by 7 %.
-PMC_SDF_VALU = 2.69884e10 + 1.24698e10 + 1.95486e8
+PMC_SDF_VALU = 3.02901e10 + 5.90839e9 + 5.69418e7
 VALU_ISSUE_PER_S = 1024 * 2.4e9 / 4
 COPY_BYTES = 1 << 30
 

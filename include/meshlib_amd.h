@@ -109,8 +109,9 @@ int mi_get_tree_builder(void);
  * reference).  A 64-query wave still walking after `steps` node visits hands its queries, with their
  * best candidates so far, to a second launch that walks the tree's 64 top subtrees for them in
  * parallel; results are identical for every budget.  0 = never split; MI_SPLIT_AUTO (the default)
- * picks per call from the tree depth and the number of waves (signed distance: 3072 steps for trees
- * of >= 18 levels, else 512 / 1024 below / above 16 k waves; projections: 64 for trees of >= 20
+ * picks per call from the tree depth and the number of waves (signed distance on trees of >= 18
+ * levels: 3072 steps below 64 k waves, 6144 below 1 M waves, no split from 1 M waves (a 512^3 grid)
+ * on; shallower trees: 512 / 1024 below / above 16 k waves; projections: 64 for trees of >= 20
  * levels, else 2048 (>= 17 levels) or 1024).  The device
  * entry points of one mesh share that mesh's workspace: calls on different streams must not overlap
  * in time. */

@@ -28,8 +28,9 @@ std::atomic<int> g_splitBudget{ MI_SPLIT_AUTO };       // signed-distance walks
 std::atomic<int> g_splitBudgetProj{ MI_SPLIT_AUTO };   // projection walks
 
 // Automatic split budgets, from sweeps on MI355X (bumpy spheres of 20 k to 5.2 M triangles, tree
-// depths 15 to 23).  Signed distance: trees of >= 18 levels hand over after 3072 steps (within a few
-// % of the best from 64^3 to 512^3); shallower trees walk less, so the 64 subtree items of a
+// depths 15 to 23).  Signed distance on trees of >= 18 levels: 3072 steps below 64 k waves, 6144
+// below 1 M (k=8 256^3: 99 -> 94.5 ms), none from 1 M waves on (512^3: k=8 521 -> 477 ms, k=9
+// 1031 -> 856 ms: enough waves to spread the long ones); shallower trees walk less, so the 64 subtree items of a
 // hand-over weigh more and an earlier hand-over spreads the few long waves (512 steps below 16 k
 // waves, 1024 above; 20 k triangles at 64^3: 2.7 -> 1.6 ms).  Projections: on trees of >= 20
 // levels nearly every wave is long and 64 steps is best (flat to within noise up to 2048 at 2^22
@@ -37,7 +38,7 @@ std::atomic<int> g_splitBudgetProj{ MI_SPLIT_AUTO };   // projection walks
 // 82 k / 328 k triangles: 36.7 / 50.8 / 54.1 ms at 64 steps, 5.0 / 10.0 / 24.0 ms at 1024-2048).
 int autoSdfBudget(int depth, int64_t waves)
 {
-    if (depth >= 18) return 3072;
+    if (depth >= 18) return waves < 65536 ? 3072 : waves < (1 << 20) ? 6144 : 0;
     return waves < 16384 ? 512 : 1024;
 }
 int autoProjBudget(int depth)
