@@ -94,6 +94,17 @@ int mi_set_memory_limit(size_t bytes);
 int mi_set_traversal(int mode);
 int mi_get_traversal(void);
 
+/* Leaf filter of the wave-cooperative signed-distance walks, process-wide (no counterpart in the
+ * reference).  A queued leaf test is dropped for the lanes where a lower bound of the triangle's
+ * distance (its plane slab and edge half-spaces, with rounding margins) already exceeds the lane's
+ * best, so results do not depend on it.  MI_LEAF_FILTER_AUTO (default) uses it for grids of at
+ * least 64 k waves (64 voxels each), where enough waves hide the load it adds to each leaf visit. */
+#define MI_LEAF_FILTER_OFF 0
+#define MI_LEAF_FILTER_ON 1
+#define MI_LEAF_FILTER_AUTO (-1)
+int mi_set_leaf_filter(int mode);
+int mi_get_leaf_filter(void);
+
 /* Where mi_mesh_create builds the AABB tree and dipoles, process-wide.  The reference builds them
  * on the host inside the first call (source/MRCuda/MRCudaFastWindingNumber.cpp:70, AABBTree ctor
  * source/MRMesh/MRAABBTree.cpp:31-60, calcDipoles source/MRMesh/MRDipole.cpp:23-71); both builders

@@ -94,6 +94,8 @@ def lib():
         "mi_set_memory_limit": (C.c_int, [SZ]),
         "mi_set_traversal": (C.c_int, [C.c_int]),
         "mi_get_traversal": (C.c_int, []),
+        "mi_set_leaf_filter": (C.c_int, [C.c_int]),
+        "mi_get_leaf_filter": (C.c_int, []),
         "mi_set_tree_builder": (C.c_int, [C.c_int]),
         "mi_get_tree_builder": (C.c_int, []),
         "mi_set_split_budget": (C.c_int, [I32, I32]),
@@ -199,6 +201,20 @@ def set_traversal(mode: str):
 def get_traversal() -> str:
     v = int(lib().mi_get_traversal())
     return {b: a for a, b in TRAVERSALS.items()}[v]
+
+
+LEAF_FILTERS = {"off": 0, "on": 1, "auto": -1}
+
+
+def set_leaf_filter(mode: str):
+    """Leaf filter of the signed-distance walks: 'on', 'off' or 'auto' (default: grids of at least
+    64 k waves).  Results are identical."""
+    _check(lib().mi_set_leaf_filter(LEAF_FILTERS[mode]))
+
+
+def get_leaf_filter() -> str:
+    v = int(lib().mi_get_leaf_filter())
+    return {b: a for a, b in LEAF_FILTERS.items()}[v]
 
 
 TREE_BUILDERS = {"device": 0, "host": 1}

@@ -278,6 +278,16 @@ int mi_set_traversal(int mode)
 
 int mi_get_traversal(void) { return static_cast<int>(mi::traversal()); }
 
+int mi_set_leaf_filter(int mode)
+{
+    if (mode != MI_LEAF_FILTER_OFF && mode != MI_LEAF_FILTER_ON && mode != MI_LEAF_FILTER_AUTO)
+        return fail(MI_E_INVALID, "unknown leaf filter mode");
+    mi::setLeafFilter(mode);
+    return MI_OK;
+}
+
+int mi_get_leaf_filter(void) { return mi::leafFilter(); }
+
 int mi_set_tree_builder(int builder)
 {
     if (builder != MI_TREE_BUILDER_DEVICE && builder != MI_TREE_BUILDER_HOST)

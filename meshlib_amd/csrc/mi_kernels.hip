@@ -45,6 +45,7 @@ constexpr int kBlock = 256;          // 4 waves
 // down two levels (one only into a leaf, where the path ends)
 constexpr int quadStack(int D) { return 3 * (D / 2 + 1); }
 constexpr int kXcds = 8;
+constexpr int64_t kLeafFilterWaves = 65536;     // signed-distance grids of at least this many waves filter leaves
 constexpr float kInv4Pi = 1.0f / (4 * 3.14159265358979f);   // INV_4PI, MRDipole.cpp:125
 
 // Mesh records are read-only while any kernel runs: reading them through the constant address
@@ -680,7 +681,7 @@ __device__ __forceinline__ float flushLeaves(const MeshDev& m, float px, float p
 // returning the best found so far -- a real candidate, so a later walk can finish from it.
 // Pruning compares slotDistFast distances with boundAbove(best), a superset of the exact tests:
 // it may add visits, never drop one.  Only the leaf tests decide the result.
-template <int D>
+template <int D, bool FILTER = true>
 __device__ __forceinline__ float coopDistanceWalk(const MeshDev& m, float px, float py, float pz, float best,
                                                   int cur, float curd, DistStack<D>* stk, int budget, bool& stopped)
 {
@@ -707,7 +708,8 @@ __device__ __forceinline__ float coopDistanceWalk(const MeshDev& m, float px, fl
         }
         if (cur < 0) {
             // lanes whose triangle may still lie closer than their best (the leaf test decides)
-            const uint64_t may = ~(uint64_t)__builtin_amdgcn_fcmpf(leafLowerBound(m, ~cur, px, py, pz), bestHi, 2);
+            const uint64_t may =
+                FILTER ? ~(uint64_t)__builtin_amdgcn_fcmpf(leafLowerBound(m, ~cur, px, py, pz), bestHi, 2) : ~0ull;
             act &= may;
             if (qn + __popcll(act) > 64) {
                 best = flushLeaves<D>(m, px, py, pz, stk, qn);
@@ -774,7 +776,7 @@ __device__ __forceinline__ float coopDistanceWalk(const MeshDev& m, float px, fl
 // 4-wide records: the minimum does not depend on the visit order, so it equals the per-lane
 // result bit for bit.  A step tests the four grandchildren of a node; the nearest active one (for
 // the lowest active lane) continues, the other active ones are pushed farthest first.
-template <int D>
+template <int D, bool FILTER = true>
 __device__ __forceinline__ float coopDistance(const MeshDev& m, float px, float py, float pz, float up,
                                               DistStack<D>* stk, int budget = 0, bool* stopped = nullptr)
 {
@@ -788,7 +790,7 @@ __device__ __forceinline__ float coopDistance(const MeshDev& m, float px, float 
     const float rootd = boxDistSq(m.rootMin[0], m.rootMin[1], m.rootMin[2], m.rootMax[0], m.rootMax[1],
                                   m.rootMax[2], px, py, pz);
     bool st = false;
-    best = coopDistanceWalk<D>(m, px, py, pz, best, m.rootRef, rootd, stk, budget, st);
+    best = coopDistanceWalk<D, FILTER>(m, px, py, pz, best, m.rootRef, rootd, stk, budget, st);
     if (stopped) *stopped = st;
     return best;
 }
@@ -1330,7 +1332,7 @@ __device__ __forceinline__ void sdfStore(const MeshDev& m, float px, float py, f
 // At most 72 VGPRs: seven waves per SIMD (the LDS of seven blocks fits a CU); the walks are bound by
 // the latency of their dependent scalar loads, and the seventh wave hides more of it than the
 // registers it costs (signed distance 105 -> 98 ms at k=8 256^3).
-template <int D>
+template <int D, bool FILTER>
 __global__ __launch_bounds__(kBlock, 7) void kSdfGridCoop(MeshDev m, int dx, int dy, int zBegin, int zEnd,
                                                        int nbx, int nby, int nbz, int nLogical, Xf xf,
                                                        DistOpts o, const float* __restrict__ bound,
@@ -1353,7 +1355,7 @@ __global__ __launch_bounds__(kBlock, 7) void kSdfGridCoop(MeshDev m, int dx, int
     float lim = o.maxDistSq;
     if (bound && valid) { const float b = bound[idx]; lim = b < lim ? b : lim; }
     bool stopped = false;
-    const float resSq = coopDistance<D>(m, px, py, pz, valid ? lim : -1.f, &stkMem[w], sc.list ? sc.budget : 0,
+    const float resSq = coopDistance<D, FILTER>(m, px, py, pz, valid ? lim : -1.f, &stkMem[w], sc.list ? sc.budget : 0,
                                         &stopped);
     if (stopped) {
         // hand the brick to the split pass with every lane's best so far
@@ -1373,7 +1375,7 @@ __global__ __launch_bounds__(kBlock, 7) void kSdfGridCoop(MeshDev m, int dx, int
 // Second launch of the split form: persistent waves take (handed-over brick, cut subtree) pairs and
 // walk that subtree for the brick's voxels from their best so far; improvements are merged by an
 // atomic minimum on the float bits (non-negative floats order like their bit patterns).
-template <int D>
+template <int D, bool FILTER>
 __global__ __launch_bounds__(kBlock) void kSdfSplit(MeshDev m, int dx, int dy, int zBegin, int zEnd, int nbx,
                                                       int nby, int nbz, Xf xf, SplitScratch sc)
 {
@@ -1399,7 +1401,7 @@ __global__ __launch_bounds__(kBlock) void kSdfSplit(MeshDev m, int dx, int dy, i
         const CutEntry E = loadCut(m, c);
         const float cd = boxDistSq(E.mn[0], E.mn[1], E.mn[2], E.mx[0], E.mx[1], E.mx[2], px, py, pz);
         bool st;
-        const float r = coopDistanceWalk<D>(m, px, py, pz, best, E.ref, cd, &stkMem[w], 0, st);
+        const float r = coopDistanceWalk<D, FILTER>(m, px, py, pz, best, E.ref, cd, &stkMem[w], 0, st);
         if (valid && r < best) atomicMin(&sc.best[(size_t)h * 64 + lane], __float_as_uint(r));
     }
 }
@@ -1870,6 +1872,13 @@ std::atomic<int> g_variant{ (int)Traversal::WaveCoop };
 void setTraversal(Traversal t) { g_variant.store((int)t); }
 Traversal traversal() { return (Traversal)g_variant.load(); }
 
+namespace {
+std::atomic<int> g_leafFilter{ -1 };
+}
+
+void setLeafFilter(int mode) { g_leafFilter.store(mode); }
+int leafFilter() { return g_leafFilter.load(); }
+
 int64_t sdfGridWaves(const int32_t dims[3], int32_t zBegin, int32_t zEnd)
 {
     const int64_t nbx = (dims[0] + 3) / 4, nby = (dims[1] + 3) / 4, nbz = (zEnd - zBegin + 3) / 4;
@@ -1890,6 +1899,11 @@ hipError_t launchSdfGrid(const MeshDev& m, const int32_t dims[3], const Xf& xf, 
     // keeps the per-lane traversal
     const bool coop = (traversal() == Traversal::WaveCoop || bound) && !(o.minDistSq > 0);
     if (bound && !coop) return hipErrorInvalidValue;
+    // The leaf filter saves instructions but adds a dependent load to every leaf visit: it pays where
+    // many waves per SIMD hide that latency and costs where a small grid leaves the SIMDs half empty
+    // (k=8: 256^3 113 -> 104.5 ms with it, 64^3 21.4 -> 23.9 ms).
+    const int filterMode = leafFilter();
+    const bool filter = filterMode < 0 ? sdfGridWaves(dims, zBegin, zEnd) >= kLeafFilterWaves : filterMode != 0;
     SplitScratch sc;
     if (coop && split && split->list && split->budget > 0 && !rerun && m.nCut > 0) {
         if ((int64_t)split->cap < sdfGridWaves(dims, zBegin, zEnd)) return hipErrorInvalidValue;
@@ -1900,13 +1914,14 @@ hipError_t launchSdfGrid(const MeshDev& m, const int32_t dims[3], const Xf& xf, 
     return dispatchDepth(stackD, [&](auto Dc) {
         constexpr int D = decltype(Dc)::value;
         if (coop) {
-            hipLaunchKernelGGL(kSdfGridCoop<D>, dim3(chunkedGrid(nLogical)), dim3(kBlock), 0, s, m, dims[0],
-                               dims[1], zBegin, zEnd, nbx, nby, nbz, nLogical, xf, o, bound, rerun, rerunCount,
-                               sc, out);
+            const auto gridKernel = filter ? kSdfGridCoop<D, true> : kSdfGridCoop<D, false>;
+            const auto splitKernel = filter ? kSdfSplit<D, true> : kSdfSplit<D, false>;
+            hipLaunchKernelGGL(gridKernel, dim3(chunkedGrid(nLogical)), dim3(kBlock), 0, s, m, dims[0], dims[1],
+                               zBegin, zEnd, nbx, nby, nbz, nLogical, xf, o, bound, rerun, rerunCount, sc, out);
             if (sc.list) {
                 const int blocks = persistentBlocks();
-                hipLaunchKernelGGL(kSdfSplit<D>, dim3(blocks), dim3(kBlock), 0, s, m, dims[0], dims[1], zBegin,
-                                   zEnd, nbx, nby, nbz, xf, sc);
+                hipLaunchKernelGGL(splitKernel, dim3(blocks), dim3(kBlock), 0, s, m, dims[0], dims[1], zBegin, zEnd,
+                                   nbx, nby, nbz, xf, sc);
                 hipLaunchKernelGGL(kSdfSplitFinish<D>, dim3(blocks), dim3(kBlock), 0, s, m, dims[0], dims[1],
                                    zBegin, zEnd, nbx, nby, nbz, xf, o, sc, out);
             }

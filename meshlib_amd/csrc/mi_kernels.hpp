@@ -44,6 +44,9 @@ static_assert(sizeof(ProjRec) == 32, "MeshProjectionResult is 32 B");
 enum class Traversal : int { PerLane = 0, WaveCoop = 1 };
 void setTraversal(Traversal t);
 Traversal traversal();
+// leaf filter of the signed-distance grid walks: 0 off, 1 on, -1 by the grid's wave count
+void setLeafFilter(int mode);
+int leafFilter();
 
 // one item of a batched launch: a signed-distance grid (gridUnits bricks of 4x4x4 voxels) and/or a
 // Morton-sorted point set (projUnits chunks of 64), on one mesh

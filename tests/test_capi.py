@@ -81,3 +81,16 @@ def test_split_budget_setting():
     assert M.lib().mi_set_split_budget(5, 16) == M.MI_E_INVALID
     M.set_split_budget(-1)
     assert M.get_split_budget("sdf") == -1 and M.get_split_budget("projections") == -1
+
+
+def test_leaf_filter_setting():
+    # automatic by default; on, off and automatic again are accepted, anything else is rejected
+    assert M.get_leaf_filter() == "auto"
+    M.set_leaf_filter("off")
+    assert M.get_leaf_filter() == "off"
+    M.set_leaf_filter("on")
+    assert M.get_leaf_filter() == "on"
+    assert M.lib().mi_set_leaf_filter(2) == M.MI_E_INVALID
+    assert M.get_leaf_filter() == "on"
+    M.set_leaf_filter("auto")
+    assert M.get_leaf_filter() == "auto"

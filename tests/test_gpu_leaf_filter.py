@@ -84,6 +84,11 @@ def test_filter_keeps_every_winner(offset, scale, degenerate):
     assert bad.size == 0, (bad[:5], g[bad[:3]], o[bad[:3]])
     dims, xf, vox = synth.grid_for_mesh(v, (40, 44, 36), expand=1.02)
     fwn = M.FastWindingNumber(gm)
-    s = fwn.calcFromGridWithDistances(dims, xf, M.DistanceToMeshOptions())
     os_ = om.sdf_grid(dims, xf)
-    assert np.array_equal(np.abs(s).view(np.uint32), np.abs(os_).view(np.uint32))
+    try:
+        for mode in ("on", "off", "auto"):      # a grid this small leaves the filter off by default
+            M.set_leaf_filter(mode)
+            s = fwn.calcFromGridWithDistances(dims, xf, M.DistanceToMeshOptions())
+            assert np.array_equal(np.abs(s).view(np.uint32), np.abs(os_).view(np.uint32)), mode
+    finally:
+        M.set_leaf_filter("auto")

@@ -24,6 +24,7 @@ def restore_budget():
     yield
     M.set_split_budget(old[0], "sdf")
     M.set_split_budget(old[1], "projections")
+    M.set_leaf_filter("auto")
 
 
 @pytest.mark.parametrize("level,bumps,grid", [(5, 8, 64), (8, 32, 96)])
@@ -43,6 +44,9 @@ def test_split_budgets_identical(level, bumps, grid, restore_budget):
         M.set_split_budget(budget)
         got = fwn.calcFromGridWithDistances(dims, xf, opts)
         assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), budget
+        M.set_leaf_filter("on" if M.get_leaf_filter() != "on" else "off")
+        got = fwn.calcFromGridWithDistances(dims, xf, opts)
+        assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), (budget, M.get_leaf_filter())
         d.fill_(-3.0)
         fwn.sdf_grid_device(dims, xf, opts, d.data_ptr())
         torch.cuda.synchronize()
