@@ -97,8 +97,9 @@ int mi_get_traversal(void);
 /* Leaf filter of the wave-cooperative signed-distance walks, process-wide (no counterpart in the
  * reference).  A queued leaf test is dropped for the lanes where a lower bound of the triangle's
  * distance (its plane slab and edge half-spaces, with rounding margins) already exceeds the lane's
- * best, so results do not depend on it.  MI_LEAF_FILTER_AUTO (default) uses it for grids of at
- * least 64 k waves (64 voxels each), where enough waves hide the load it adds to each leaf visit. */
+ * best, so results do not depend on it.  MI_LEAF_FILTER_AUTO (default) uses it for grids of more
+ * than 2^(depth - 8) waves of 64 voxels (depth = tree levels): on deep trees a small grid leaves too
+ * few waves to hide the load it adds to each leaf visit. */
 #define MI_LEAF_FILTER_OFF 0
 #define MI_LEAF_FILTER_ON 1
 #define MI_LEAF_FILTER_AUTO (-1)

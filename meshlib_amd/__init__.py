@@ -207,8 +207,8 @@ LEAF_FILTERS = {"off": 0, "on": 1, "auto": -1}
 
 
 def set_leaf_filter(mode: str):
-    """Leaf filter of the signed-distance walks: 'on', 'off' or 'auto' (default: grids of at least
-    64 k waves).  Results are identical."""
+    """Leaf filter of the signed-distance walks: 'on', 'off' or 'auto' (default: grids of more than
+    2^(depth - 8) waves of 64 voxels).  Results are identical."""
     _check(lib().mi_set_leaf_filter(LEAF_FILTERS[mode]))
 
 

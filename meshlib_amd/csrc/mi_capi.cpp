@@ -475,6 +475,7 @@ int mi_mesh_create(const float* verts, int32_t nv, const int32_t* tris, int32_t 
         return rc;
     }
     m->dev.numFaces = nf;
+    m->dev.depth = m->depth;
     *out = m;
     return MI_OK;
 }

@@ -45,7 +45,6 @@ constexpr int kBlock = 256;          // 4 waves
 // down two levels (one only into a leaf, where the path ends)
 constexpr int quadStack(int D) { return 3 * (D / 2 + 1); }
 constexpr int kXcds = 8;
-constexpr int64_t kLeafFilterWaves = 65536;     // signed-distance grids of at least this many waves filter leaves
 constexpr float kInv4Pi = 1.0f / (4 * 3.14159265358979f);   // INV_4PI, MRDipole.cpp:125
 
 // Mesh records are read-only while any kernel runs: reading them through the constant address
@@ -1899,11 +1898,14 @@ hipError_t launchSdfGrid(const MeshDev& m, const int32_t dims[3], const Xf& xf, 
     // keeps the per-lane traversal
     const bool coop = (traversal() == Traversal::WaveCoop || bound) && !(o.minDistSq > 0);
     if (bound && !coop) return hipErrorInvalidValue;
-    // The leaf filter saves instructions but adds a dependent load to every leaf visit: it pays where
-    // many waves per SIMD hide that latency and costs where a small grid leaves the SIMDs half empty
-    // (k=8: 256^3 113 -> 104.5 ms with it, 64^3 21.4 -> 23.9 ms).
+    // The leaf filter saves instructions but adds a dependent load to every leaf visit.  It pays unless
+    // a deep tree meets a small grid: then few waves walk long chains and nothing hides that load.
+    // Measured on / off (bumpy spheres): k=5 64^3 0.89, k=7 128^3 0.87, k=8 96^3 0.96, k=8 256^3 0.87,
+    // k=9 192^3 0.99 -- but k=8 64^3 (4 k waves) 1.11 and k=9 128^3 (32 k waves) 1.05; on where the grid
+    // has more than 2^(depth - 8) waves.
     const int filterMode = leafFilter();
-    const bool filter = filterMode < 0 ? sdfGridWaves(dims, zBegin, zEnd) >= kLeafFilterWaves : filterMode != 0;
+    const int64_t waves = sdfGridWaves(dims, zBegin, zEnd);
+    const bool filter = filterMode < 0 ? waves > ((int64_t)1 << (m.depth > 8 ? m.depth - 8 : 0)) : filterMode != 0;
     SplitScratch sc;
     if (coop && split && split->list && split->budget > 0 && !rerun && m.nCut > 0) {
         if ((int64_t)split->cap < sdfGridWaves(dims, zBegin, zEnd)) return hipErrorInvalidValue;

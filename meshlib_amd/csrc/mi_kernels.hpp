@@ -30,6 +30,7 @@ struct MeshDev {
     float rootMin[3], rootMax[3];
     int32_t rootRef;
     int32_t numFaces;
+    int32_t depth;         // levels of the tree
 };
 
 struct Xf { float m[12]; };   // row-major A then b (AffineXf3f)

@@ -86,7 +86,7 @@ def test_filter_keeps_every_winner(offset, scale, degenerate):
     fwn = M.FastWindingNumber(gm)
     os_ = om.sdf_grid(dims, xf)
     try:
-        for mode in ("on", "off", "auto"):      # a grid this small leaves the filter off by default
+        for mode in ("on", "off", "auto"):      # auto: by the grid's size against the tree's depth
             M.set_leaf_filter(mode)
             s = fwn.calcFromGridWithDistances(dims, xf, M.DistanceToMeshOptions())
             assert np.array_equal(np.abs(s).view(np.uint32), np.abs(os_).view(np.uint32)), mode
