@@ -1354,8 +1354,8 @@ __global__ __launch_bounds__(kBlock, 7) void kSdfGridCoop(MeshDev m, int dx, int
     float lim = o.maxDistSq;
     if (bound && valid) { const float b = bound[idx]; lim = b < lim ? b : lim; }
     bool stopped = false;
-    const float resSq = coopDistance<D, FILTER>(m, px, py, pz, valid ? lim : -1.f, &stkMem[w], sc.list ? sc.budget : 0,
-                                        &stopped);
+    const float resSq = coopDistance<D, FILTER>(m, px, py, pz, valid ? lim : -1.f, &stkMem[w],
+                                                sc.list ? sc.budget : 0, &stopped);
     if (stopped) {
         // hand the brick to the split pass with every lane's best so far
         uint32_t slot = 0;
@@ -1728,9 +1728,10 @@ __device__ float roundDownF(double x)
     return f;
 }
 
-// One leaf slot's plane record (leafMayWin): normal and edge normals in FP64, rounded to float,
+// One leaf slot's plane record (leafLowerBound): normal and edge normals in FP64, rounded to float,
 // then the offsets of those float planes over the three vertices in FP64 (products of floats are
-// exact there), widened by r * 2^-40 and rounded outwards.
+// exact there), widened by r * 2^-40 and rounded outwards.  Layout, 5 float4: (n, lo), (hi, m0),
+// (max m0.v, m1), (max m1.v, m2), (max m2.v, r, 0, 0).
 __global__ void kBuildPrisms(const float4* __restrict__ tri, int n, float4* __restrict__ prism)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
