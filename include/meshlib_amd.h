@@ -137,18 +137,24 @@ int32_t mi_get_split_budget(int32_t walk);
 
 /* Cuda::FastWindingNumber::prepareData_ (source/MRCuda/MRCudaFastWindingNumber.cpp:46-85)
  * and Cuda::PointsToMeshProjector::updateMeshData (source/MRCuda/MRCudaPointsToMeshProjector.cpp:31-50):
- * verts float[nv*3], tris int32[nf*3] (getTriangulation order).  Builds the AABB tree
- * (source/MRMesh/MRAABBTree.cpp:31-60) and dipoles (source/MRMesh/MRDipole.cpp:23-71) with the
- * selected builder and keeps the flattened tree on the current HIP device.  At most 2^30 faces. */
+ * verts float[nv*3], tris int32[nf*3] (getTriangulation order: nf = faceSize(), a deleted face is
+ * {-1,-1,-1}, source/MRMesh/MRMeshTopology.cpp:430-441).  Builds the AABB tree over the valid faces,
+ * leaves keeping their face ids (source/MRMesh/MRAABBTree.cpp:31-60), and the dipoles
+ * (source/MRMesh/MRDipole.cpp:23-71) with the selected builder and keeps the flattened tree on the
+ * current HIP device.  A face with any negative index is a deleted face (skipped, as the reference's
+ * GPU kernels do, source/MRCuda/MRCudaFastWindingNumber.cu:168); an index >= nv is MI_E_INVALID.
+ * Face ids in results (projection records, skipFace, self-intersection flags) index the caller's
+ * array.  At most 2^30 faces. */
 int  mi_mesh_create(const float* verts, int32_t nv, const int32_t* tris, int32_t nf, mi_mesh** out);
 void mi_mesh_destroy(mi_mesh* m);
+/* tree nodes: 2 * (valid faces) - 1, or 0 */
 int  mi_mesh_num_nodes(const mi_mesh* m);
 /* exports the tree / dipoles in the reference layout (AABBTree::nodes(), Mesh::getDipoles()) */
 int  mi_mesh_get_tree(const mi_mesh* m, mi_tree_node* nodes, mi_dipole* dipoles);
 /* depth of the tree (longest root-to-leaf path, in edges) */
 int  mi_mesh_depth(const mi_mesh* m);
 /* the same host tree + dipole build without any device (AABBTree ctor + calcDipoles):
- * fills nodes[2*nf-1] and dipoles[2*nf-1] (either may be NULL) and *depth */
+ * fills nodes[2*k-1] and dipoles[2*k-1], k = the valid faces of tris (either may be NULL), and *depth */
 int  mi_build_tree_host(const float* verts, int32_t nv, const int32_t* tris, int32_t nf,
                         mi_tree_node* nodes, mi_dipole* dipoles, int32_t* depth);
 
@@ -159,7 +165,8 @@ int  mi_build_tree_host(const float* verts, int32_t nv, const int32_t* tris, int
 int mi_fwn_calc_from_vector(mi_mesh* m, const float* pts, int64_t n, float beta, int32_t skipFace,
                             float* res, mi_progress_fn cb, void* user);
 /* IFastWindingNumber::calcSelfIntersections (MRFastWindingNumber.h:31;
- * GPU MRCudaFastWindingNumber.cpp:129-173): res[f] = 1 if wn(centroid f, skip f) outside [0,1] */
+ * GPU MRCudaFastWindingNumber.cpp:129-173): res[f] = 1 if wn(centroid f, skip f) outside [0,1];
+ * res has one entry per face of the array passed to mi_mesh_create (deleted faces: 0) */
 int mi_fwn_calc_self_intersections(mi_mesh* m, float beta, uint8_t* res, mi_progress_fn cb, void* user);
 /* IFastWindingNumber::calcFromGrid (MRFastWindingNumber.h:38; GPU MRCudaFastWindingNumber.cpp:175-228);
  * xf = gridToMeshXf as float[12]: row-major A then b; voxel ids x-fastest */

@@ -180,7 +180,8 @@ def build_tree_host(verts, tris):
     """Host-only tree + dipole build (no device): (nodes, dipoles, depth) in the reference layout."""
     v = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1, 3)
     t = np.ascontiguousarray(tris, dtype=np.int32).reshape(-1, 3)
-    n = max(2 * t.shape[0] - 1, 0)
+    valid = int(np.count_nonzero((t >= 0).all(axis=1)))     # deleted faces ({-1,-1,-1}) get no leaf
+    n = max(2 * valid - 1, 0)
     nodes = np.zeros(n, NODE_DTYPE)
     dips = np.zeros(n, DIPOLE_DTYPE)
     depth = C.c_int32(0)

@@ -1,7 +1,8 @@
 """Build recipe of the native library ``meshlib_amd/libmeshlib_amd.so`` (gfx950).
 
 ``python -m meshlib_amd.build`` compiles the host tree builder, the HIP kernels and the
-C-ABI into one shared library in-tree.  Flags:
+C-ABI into one shared library in-tree: every source to its own object (in parallel, under
+``meshlib_amd/_build/``), then one link.  Flags:
 
 * ``-ffp-contract=off``: no fused multiply-add contraction, so every float/double
   expression rounds exactly like the reference CPU build (bit-identical distances);
@@ -14,14 +15,22 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmeshlib_amd.so")
 ARCH = os.environ.get("MESHLIB_AMD_ARCH", "gfx950")
 
-SOURCES = ["mi_tree.cpp", "mi_tree_gpu.hip", "mi_kernels.hip", "mi_capi.cpp", "mi_mlp.hip", "mi_mlp_capi.cpp"]
+SOURCES = ["mi_tree.cpp", "mi_tree_gpu.hip", "mi_kernels.hip", "mi_kernels_band.hip", "mi_capi.cpp",
+           "mi_mlp.hip", "mi_mlp_capi.cpp"]
+
+FLAGS = [f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
+         "-ffp-contract=off", "-fhip-fp32-correctly-rounded-divide-sqrt",
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         "-Wall", "-Wno-unused-function"]
 
 
 def hipcc() -> str:
@@ -31,20 +40,32 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
-def build(verbose: bool = False) -> str:
-    tmp = LIB + ".tmp"
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
-           "-ffp-contract=off", "-fhip-fp32-correctly-rounded-divide-sqrt",
-           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-           "-Wall", "-Wno-unused-function",
-           "-I", os.path.join(ROOT, "include"), "-I", CSRC,
-           *[os.path.join(CSRC, s) for s in SOURCES], "-o", tmp, "-lpthread"]
+def _compile(src: str, verbose: bool) -> str:
+    obj = os.path.join(OBJ, src + ".o")
+    cmd = [hipcc(), *FLAGS, "-I", os.path.join(ROOT, "include"), "-I", CSRC, "-c",
+           os.path.join(CSRC, src), "-o", obj]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed")
+        raise RuntimeError(f"hipcc failed on {src}")
+    return obj
+
+
+def build(verbose: bool = False) -> str:
+    os.makedirs(OBJ, exist_ok=True)
+    jobs = max(1, min(len(SOURCES), os.cpu_count() or 1, 16))
+    with ThreadPoolExecutor(jobs) as ex:
+        objs = list(ex.map(lambda s: _compile(s, verbose), SOURCES))
+    tmp = LIB + ".tmp"
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp, "-lpthread"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("link failed")
     os.replace(tmp, LIB)
     return LIB
 

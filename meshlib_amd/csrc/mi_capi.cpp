@@ -177,7 +177,8 @@ int mi_internal_fail(int code, const char* msg) { return fail(code, msg); }
 
 struct mi_mesh {
     int device = 0;
-    int32_t nv = 0, nf = 0;
+    int32_t nv = 0, nf = 0;       // nf: faces in the tree (the valid ones)
+    int32_t faceSize = 0;         // entries of the caller's triangle array, deleted faces included
     int32_t numNodes = 0, depth = 0;
     mi::HostTree tree;     // host build: filled at creation; device build: downloaded on request
     DevBuf treeDev;        // device build: nodes + dipoles in the reference layout
@@ -377,16 +378,56 @@ void setRoot(mi_mesh* m, const mi::Box3& rootBox, int32_t rootRef)
     m->dev.rootRef = rootRef;
 }
 
-// tree built by the host builder, flattened on the host and uploaded
-int createHostBuilt(mi_mesh* m, const float* verts, const int32_t* tris)
+// The valid faces of a triangle array in getTriangulation order.  MeshTopology::getTriangulation
+// (source/MRMesh/MRMeshTopology.cpp:430-441) returns faceSize() entries with {-1,-1,-1} for a deleted
+// face, and the reference's tree covers the valid faces only, keeping their FaceIds as leaf ids
+// (source/MRMesh/MRAABBTree.cpp:31-49).  A face with any negative index counts as deleted, as in the
+// reference's GPU kernels (source/MRCuda/MRCudaFastWindingNumber.cu:168); an index >= nv is an error.
+// `packed` / `ids` stay empty when every face is valid.
+struct ValidFaces {
+    std::vector<int32_t> packed, ids;
+    int32_t count = 0;
+};
+
+int validFaces(const int32_t* tris, int32_t nf, int32_t nv, ValidFaces& out)
 {
-    for (int64_t i = 0; i < 3 * (int64_t)m->nf; ++i)
-        if (tris[i] < 0 || tris[i] >= m->nv) return fail(MI_E_INVALID, "triangle vertex index out of range");
+    out = ValidFaces{};
+    int64_t bad = 0, deleted = 0;
+    for (int32_t f = 0; f < nf; ++f) {
+        const int32_t* t = tris + 3 * (int64_t)f;
+        if (t[0] < 0 || t[1] < 0 || t[2] < 0) ++deleted;
+        else if (t[0] >= nv || t[1] >= nv || t[2] >= nv) ++bad;
+    }
+    if (bad) return fail(MI_E_INVALID, "triangle vertex index out of range");
+    out.count = nf - (int32_t)deleted;
+    if (!deleted) return MI_OK;
     try {
-        mi::buildHostTree(verts, m->nv, tris, m->nf, m->tree);
+        out.packed.reserve(3 * (size_t)out.count);
+        out.ids.reserve((size_t)out.count);
+        for (int32_t f = 0; f < nf; ++f) {
+            const int32_t* t = tris + 3 * (int64_t)f;
+            if (t[0] < 0 || t[1] < 0 || t[2] < 0) continue;
+            out.packed.insert(out.packed.end(), t, t + 3);
+            out.ids.push_back(f);
+        }
+    } catch (const std::bad_alloc&) {
+        return fail(MI_E_NOMEM, "out of host memory packing the valid faces");
+    }
+    return MI_OK;
+}
+
+// tree built by the host builder, flattened on the host and uploaded
+int createHostBuilt(mi_mesh* m, const float* verts, const int32_t* tris, const ValidFaces& vf)
+{
+    try {
+        mi::buildHostTree(verts, m->nv, vf.ids.empty() ? tris : vf.packed.data(), m->nf, m->tree);
     } catch (const std::bad_alloc&) {
         return fail(MI_E_NOMEM, "out of host memory building the tree");
     }
+    // leaves name packed indices; the flattened records then read the caller's array by face id
+    if (!vf.ids.empty())
+        for (mi::TreeNode& n : m->tree.nodes)
+            if (n.r < 0) n.l = vf.ids[(size_t)n.l];
     m->depth = m->tree.depth;
     m->stackD = mi::stackDepthFor(m->depth);
     if (m->stackD == 0) return fail(MI_E_LIMIT, "tree deeper than 32 levels");
@@ -412,7 +453,7 @@ int createHostBuilt(mi_mesh* m, const float* verts, const int32_t* tris)
 }
 
 // tree, dipoles and the flattened layout built by the device builder from uploaded mesh arrays
-int createDeviceBuilt(mi_mesh* m, const float* verts, const int32_t* tris)
+int createDeviceBuilt(mi_mesh* m, const float* verts, const int32_t* tris, const ValidFaces& vf)
 {
     if (m->nf == 0) return MI_OK;
     const size_t nn = 2 * (size_t)m->nf - 1;
@@ -420,12 +461,17 @@ int createDeviceBuilt(mi_mesh* m, const float* verts, const int32_t* tris)
     MI_HIP(m->arena.alloc(L.total()));
     bindArena(m, L);
     MI_HIP(m->treeDev.alloc(2 * align256(nn * sizeof(mi::TreeNode))));
-    DevBuf dv, dt, scratch;
+    DevBuf dv, dt, scratch, ids;
     MI_HIP(dv.alloc(std::max<size_t>((size_t)m->nv * 12, 16)));
     MI_HIP(dt.alloc((size_t)m->nf * 12));
     MI_HIP(scratch.alloc(mi::deviceTreeScratchBytes(m->nf)));
     if (m->nv > 0) MI_HIP(hipMemcpyAsync(dv.p, verts, (size_t)m->nv * 12, hipMemcpyHostToDevice, m->stream));
-    MI_HIP(hipMemcpyAsync(dt.p, tris, (size_t)m->nf * 12, hipMemcpyHostToDevice, m->stream));
+    MI_HIP(hipMemcpyAsync(dt.p, vf.ids.empty() ? tris : vf.packed.data(), (size_t)m->nf * 12, hipMemcpyHostToDevice,
+                          m->stream));
+    if (!vf.ids.empty()) {
+        MI_HIP(ids.alloc((size_t)m->nf * 4));
+        MI_HIP(hipMemcpyAsync(ids.p, vf.ids.data(), (size_t)m->nf * 4, hipMemcpyHostToDevice, m->stream));
+    }
     char* base = m->arena.as<char>();
     mi::DeviceTreeOut o;
     o.nodes = m->treeDev.as<mi::TreeNode>();
@@ -436,7 +482,7 @@ int createDeviceBuilt(mi_mesh* m, const float* verts, const int32_t* tris)
     o.dleaf = reinterpret_cast<float4*>(base + L.dnode + L.tri + L.dint);
     mi::DeviceTreeInfo info;
     const hipError_t e = mi::buildDeviceTree(dv.as<float>(), m->nv, dt.as<int32_t>(), m->nf, scratch.p, scratch.bytes,
-                                             o, info, m->stream);
+                                             o, info, m->stream, ids.as<int32_t>());
     if (e == hipErrorInvalidValue) return fail(MI_E_INVALID, "triangle vertex index out of range");
     MI_HIP(e);
     m->depth = info.depth;
@@ -458,23 +504,26 @@ int mi_mesh_create(const float* verts, int32_t nv, const int32_t* tris, int32_t 
     if (nv < 0 || nf < 0 || (nv > 0 && !verts) || (nf > 0 && !tris))
         return fail(MI_E_INVALID, "invalid mesh arrays");
     if (nf > (1 << 30)) return fail(MI_E_LIMIT, "more than 2^30 faces");
+    ValidFaces vf;
+    if (int rc = validFaces(tris, nf, nv, vf)) return rc;
     mi_mesh* m = new (std::nothrow) mi_mesh();
     if (!m) return fail(MI_E_NOMEM, "out of host memory");
     m->nv = nv;
-    m->nf = nf;
-    m->numNodes = nf > 0 ? 2 * nf - 1 : 0;
+    m->nf = vf.count;
+    m->faceSize = nf;
+    m->numNodes = vf.count > 0 ? 2 * vf.count - 1 : 0;
     if (hipGetDevice(&m->device) != hipSuccess) { delete m; return fail(MI_E_HIP, "AMD GPU error: no device"); }
     if (hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess) {
         delete m;
         return fail(MI_E_HIP, "AMD GPU error: stream creation failed");
     }
-    const int rc = g_treeBuilder.load() == MI_TREE_BUILDER_HOST ? createHostBuilt(m, verts, tris)
-                                                                 : createDeviceBuilt(m, verts, tris);
+    const int rc = g_treeBuilder.load() == MI_TREE_BUILDER_HOST ? createHostBuilt(m, verts, tris, vf)
+                                                                 : createDeviceBuilt(m, verts, tris, vf);
     if (rc != MI_OK) {
         delete m;
         return rc;
     }
-    m->dev.numFaces = nf;
+    m->dev.numFaces = m->nf;
     m->dev.depth = m->depth;
     *out = m;
     return MI_OK;
@@ -496,14 +545,17 @@ int mi_build_tree_host(const float* verts, int32_t nv, const int32_t* tris, int3
 {
     if (nv < 0 || nf < 0 || (nv > 0 && !verts) || (nf > 0 && !tris))
         return fail(MI_E_INVALID, "invalid mesh arrays");
-    for (int64_t i = 0; i < 3 * (int64_t)nf; ++i)
-        if (tris[i] < 0 || tris[i] >= nv) return fail(MI_E_INVALID, "triangle vertex index out of range");
+    ValidFaces vf;
+    if (int rc = validFaces(tris, nf, nv, vf)) return rc;
     mi::HostTree t;
     try {
-        mi::buildHostTree(verts, nv, tris, nf, t);
+        mi::buildHostTree(verts, nv, vf.ids.empty() ? tris : vf.packed.data(), vf.count, t);
     } catch (const std::bad_alloc&) {
         return fail(MI_E_NOMEM, "out of host memory building the tree");
     }
+    if (!vf.ids.empty())
+        for (mi::TreeNode& n : t.nodes)
+            if (n.r < 0) n.l = vf.ids[(size_t)n.l];
     if (nodes && !t.nodes.empty()) std::memcpy(nodes, t.nodes.data(), t.nodes.size() * sizeof(mi_tree_node));
     if (dipoles && !t.dipoles.empty())
         std::memcpy(dipoles, t.dipoles.data(), t.dipoles.size() * sizeof(mi_dipole));
@@ -924,15 +976,19 @@ int mi_fwn_calc_from_vector(mi_mesh* m, const float* pts, int64_t n, float beta,
 
 int mi_fwn_calc_self_intersections(mi_mesh* m, float beta, uint8_t* res, mi_progress_fn cb, void* user)
 {
-    if (!m || (m->nf > 0 && !res)) return fail(MI_E_INVALID, "invalid arguments");
+    if (!m || (m->faceSize > 0 && !res)) return fail(MI_E_INVALID, "invalid arguments");
     std::lock_guard<std::mutex> lk(m->mu);
     MI_HIP(hipSetDevice(m->device));
     if (!report(cb, user, 0.5f)) return fail(MI_E_CANCELED, "Operation was canceled");
-    if (m->nf == 0) return MI_OK;
+    if (m->faceSize == 0) return MI_OK;
+    // one flag per entry of the caller's triangle array; deleted faces stay 0 (the reference sets bits
+    // of valid faces only, source/MRMesh/MRFastWindingNumber.cpp:40-52)
+    const int words = (int)(((size_t)m->faceSize + 3) / 4);
     DevBuf out;
-    MI_HIP(out.alloc((size_t)m->nf));
+    MI_HIP(out.alloc((size_t)words * 4));
+    MI_HIP(mi::fillU32(out.as<uint32_t>(), words, 0u, m->stream));
     MI_HIP(mi::launchSelfIntersections(m->dev, beta, out.as<uint8_t>(), m->stackD, m->stream));
-    MI_HIP(hipMemcpyAsync(res, out.p, (size_t)m->nf, hipMemcpyDeviceToHost, m->stream));
+    MI_HIP(hipMemcpyAsync(res, out.p, (size_t)m->faceSize, hipMemcpyDeviceToHost, m->stream));
     MI_HIP(hipStreamSynchronize(m->stream));
     if (!report(cb, user, 1.0f)) return fail(MI_E_CANCELED, "Operation was canceled");
     return MI_OK;

@@ -368,6 +368,21 @@ __global__ void kFlatten(int32_t nn, const TreeNode* __restrict__ nodes, const D
     }
 }
 
+// A tree built over the valid faces of a mesh with deleted faces names the packed face index in its
+// leaves; the reference's leaves hold the real FaceId (source/MRMesh/MRAABBTree.cpp:35-49).  Packing
+// keeps the face order, so the tree (whose ties fall to the face index) is the same; only the ids
+// written to the leaf nodes and the triangle records are mapped back.
+__global__ void kRemapFaces(int32_t nn, TreeNode* __restrict__ nodes, const int32_t* __restrict__ leafFlag,
+                            const int32_t* __restrict__ leafBefore, float4* __restrict__ tri,
+                            const int32_t* __restrict__ faceIds)
+{
+    const int32_t i = blockIdx.x * kB + threadIdx.x;
+    if (i >= nn || !leafFlag[i]) return;
+    const int32_t f = faceIds[nodes[i].l];
+    nodes[i].l = f;
+    tri[3 * (int64_t)leafBefore[i] + 2].y = __int_as_float(f);
+}
+
 inline size_t al(size_t b) { return (b + 255) & ~size_t(255); }
 inline int bitsFor(uint64_t x) { int b = 1; while (b < 64 && (x >> b)) ++b; return b; }
 inline unsigned gridFor(int64_t n) { return (unsigned)((n + kB - 1) / kB); }
@@ -462,7 +477,8 @@ size_t deviceTreeScratchBytes(int32_t nf)
 }
 
 hipError_t buildDeviceTree(const float* d_verts, int32_t nv, const int32_t* d_tris, int32_t nf, void* scratch,
-                           size_t scratchBytes, const DeviceTreeOut& out, DeviceTreeInfo& info, hipStream_t st)
+                           size_t scratchBytes, const DeviceTreeOut& out, DeviceTreeInfo& info, hipStream_t st,
+                           const int32_t* d_faceIds)
 {
     info = DeviceTreeInfo{};
     if (nf <= 0) return hipSuccess;
@@ -533,6 +549,7 @@ hipError_t buildDeviceTree(const float* d_verts, int32_t nv, const int32_t* d_tr
     TG_HIP(hipcub::DeviceScan::ExclusiveSum(S.temp, tb, S.leafFlag, S.leafBefore, nn, st));
     kFlatten<<<gridFor(nn), kB, 0, st>>>(nn, out.nodes, out.dipoles, S.leafFlag, S.leafBefore, d_verts, d_tris,
                                          out.dnode, out.tri, out.dint, out.dleaf);
+    if (d_faceIds) kRemapFaces<<<gridFor(nn), kB, 0, st>>>(nn, out.nodes, S.leafFlag, S.leafBefore, out.tri, d_faceIds);
     TG_HIP(hipGetLastError());
     TreeNode root;
     TG_HIP(hipMemcpyAsync(&root, out.nodes, sizeof(TreeNode), hipMemcpyDeviceToHost, st));

@@ -33,7 +33,10 @@ size_t deviceTreeScratchBytes(int32_t nf);
 
 // d_verts float[nv*3], d_tris int32[nf*3] (device).  Returns hipErrorInvalidValue when a triangle
 // names a vertex outside [0, nv).  Synchronises `s` (the level loop reads each level's size).
+// d_faceIds (optional, device int32[nf]): the face id of each packed triangle, written to the leaf
+// nodes and triangle records instead of the packed index (meshes with deleted faces).
 hipError_t buildDeviceTree(const float* d_verts, int32_t nv, const int32_t* d_tris, int32_t nf, void* scratch,
-                           size_t scratchBytes, const DeviceTreeOut& out, DeviceTreeInfo& info, hipStream_t s);
+                           size_t scratchBytes, const DeviceTreeOut& out, DeviceTreeInfo& info, hipStream_t s,
+                           const int32_t* d_faceIds = nullptr);
 
 } // namespace mi

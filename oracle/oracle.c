@@ -227,15 +227,31 @@ static void face_box(const OrMesh* m, int32_t f, OrBox* box)
 
 typedef struct { int32_t root, first, num; } Subtree;
 
+/* a face of the triangulation that is not deleted: getTriangulation (MRMeshTopology.cpp:430-441)
+ * writes only valid faces, the others keep {-1,-1,-1}; the reference's GPU kernels test any index
+ * < 0 (MRCudaFastWindingNumber.cu:168) */
+static int face_valid(const OrMesh* m, int32_t f)
+{
+    const int32_t* t = m->tris + 3 * (int64_t)f;
+    return t[0] >= 0 && t[1] >= 0 && t[2] >= 0;
+}
+
 /* AABBTree(MeshPart) + AABBTreeMaker::construct/makeNode_/makeSubtree_
- * (MRAABBTree.cpp:31-60, MRAABBTreeMaker.hpp:76-165); serial, same result */
+ * (MRAABBTree.cpp:31-60, MRAABBTreeMaker.hpp:76-165); serial, same result.  The leaves are the
+ * valid faces with their own ids (MRAABBTree.cpp:35-49, the "not packed" branch). */
 static void build_tree(OrMesh* m)
 {
-    const int32_t nf = m->nf;
+    int32_t nf = 0;
+    for (int32_t f = 0; f < m->nf; ++f) nf += face_valid(m, f);
     m->nnodes = nf > 0 ? 2 * nf - 1 : 0;
     if (nf <= 0) { m->nodes = NULL; return; }
     BoxedLeaf* leaves = (BoxedLeaf*)malloc(sizeof(BoxedLeaf) * (size_t)nf);
-    for (int32_t f = 0; f < nf; ++f) { leaves[f].leafId = f; face_box(m, f, &leaves[f].box); }
+    for (int32_t f = 0, i = 0; f < m->nf; ++f) {
+        if (!face_valid(m, f)) continue;
+        leaves[i].leafId = f;
+        face_box(m, f, &leaves[i].box);
+        ++i;
+    }
     m->nodes = (OrNode*)malloc(sizeof(OrNode) * (size_t)m->nnodes);
     Subtree* stack = (Subtree*)malloc(sizeof(Subtree) * 128);
     int sp = 0;
@@ -738,6 +754,8 @@ static void self_body(void* c, int64_t b, int64_t e, OrCounters* cnt)
     const float third = 1 / 3.0f;
     for (int64_t f = b; f < e; ++f) {
         float a[3], bb[3], cc[3], q[3];
+        x->out[f] = 0;
+        if (!face_valid(x->m, (int32_t)f)) continue;     /* BitSetParallelFor over valid faces */
         tri_points(x->m, (int32_t)f, a, bb, cc);
         for (int k = 0; k < 3; ++k) q[k] = third * ((a[k] + bb[k]) + cc[k]);
         float wn = calc_fwn(x->m, q, x->beta, (int32_t)f, cnt);
@@ -862,6 +880,7 @@ static void bf_nearest_body(void* c, int64_t b, int64_t e, OrCounters* cnt)
         double best = INFINITY;
         for (int32_t f = 0; f < x->nf; ++f) {
             const int32_t* t = x->tris + 3 * (int64_t)f;
+            if (t[0] < 0 || t[1] < 0 || t[2] < 0) continue;
             double a[3], bb[3], cc[3], pr[3], ba[2];
             for (int k = 0; k < 3; ++k) {
                 a[k] = x->verts[3 * (int64_t)t[0] + k];
@@ -892,6 +911,7 @@ static void bf_winding_body(void* c, int64_t b, int64_t e, OrCounters* cnt)
         double s = 0;
         for (int32_t f = 0; f < x->nf; ++f) {
             const int32_t* t = x->tris + 3 * (int64_t)f;
+            if (t[0] < 0 || t[1] < 0 || t[2] < 0) continue;
             double a[3], bb[3], cc[3];
             for (int k = 0; k < 3; ++k) {
                 a[k] = x->verts[3 * (int64_t)t[0] + k];

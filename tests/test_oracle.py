@@ -207,11 +207,100 @@ def test_dipoles_invariants(small_mesh):
     assert np.allclose(dips["area"][leaf], area[fl], rtol=1e-6)
     cen = (a + b + c) / 3
     assert np.allclose(dips["pos"][leaf], cen[fl], atol=1e-6)
-    # internal centre = area-weighted centre of its leaves; rr reaches a box corner
-    for i in np.nonzero(~leaf)[0][:50]:
-        pass
+    # internal nodes (source/MRMesh/MRDipole.cpp:47-70): area = the sum of the areas of the leaves
+    # below, pos = their area-weighted centre, dirArea = the sum of their directed areas.  The leaves
+    # below node i are the leaf nodes of its pre-order range [i, end_i), end_i = end of its right
+    # child's range (found from the r links alone), summed in FP64 from the triangles themselves.
+    n = len(nodes)
+    end = np.zeros(n, np.int64)
+    for i in range(n - 1, -1, -1):
+        end[i] = i + 1 if nodes["r"][i] < 0 else end[nodes["r"][i]]
+    assert end[0] == n
+    dA = 0.5 * cr
+    w_area = np.zeros(n + 1)
+    w_pos = np.zeros((n + 1, 3))
+    w_dir = np.zeros((n + 1, 3))
+    w_area[1:][leaf] = area[fl]
+    w_pos[1:][leaf] = area[fl, None] * cen[fl]
+    w_dir[1:][leaf] = dA[fl]
+    c_area, c_pos, c_dir = np.cumsum(w_area), np.cumsum(w_pos, 0), np.cumsum(w_dir, 0)
+    inner = np.nonzero(~leaf)[0]
+    s_area = c_area[end[inner]] - c_area[inner]
+    s_pos = (c_pos[end[inner]] - c_pos[inner]) / s_area[:, None]
+    s_dir = c_dir[end[inner]] - c_dir[inner]
+    scale = float(np.abs(v).max())
+    assert np.allclose(dips["area"][inner], s_area, rtol=2e-5)
+    assert np.all(np.abs(dips["pos"][inner] - s_pos) <= 2e-5 * scale)
+    assert np.all(np.abs(dips["dirArea"][inner] - s_dir) <= 2e-5 * s_area[:, None] + 1e-7)
+    # rr reaches the farthest box corner
     corners_max = np.maximum((dips["pos"] - nodes["min"]) ** 2, (dips["pos"] - nodes["max"]) ** 2).sum(1)
     assert np.allclose(dips["rr"], corners_max, rtol=1e-6)
+
+
+def test_dipole_check_catches_planted_mistakes(small_mesh):
+    # the internal-node check above must fail for an unweighted centre or a child dropped from a sum
+    v, f, m = small_mesh
+    nodes, dips = m.nodes(), m.dipoles()
+    leaf = nodes["r"] < 0
+    inner = np.nonzero(~leaf)[0]
+    l, r = nodes["l"][inner], nodes["r"][inner]
+    al, ar = dips["area"][l].astype(np.float64), dips["area"][r].astype(np.float64)
+    pl, pr = dips["pos"][l].astype(np.float64), dips["pos"][r].astype(np.float64)
+    right_pos = (al[:, None] * pl + ar[:, None] * pr) / (al + ar)[:, None]
+    wrong_pos = 0.5 * (pl + pr)                                                       # not area-weighted
+    assert np.max(np.abs(wrong_pos - right_pos)) > 2e-5 * float(np.abs(v).max())
+    wrong_area = al                                                                   # right child dropped
+    assert np.all(np.abs(wrong_area - dips["area"][inner]) > 2e-5 * dips["area"][inner])
+
+
+def _delete_faces(f, frac, seed):
+    rng = np.random.default_rng(seed)
+    fd = f.copy()
+    gone = rng.random(f.shape[0]) < frac
+    fd[gone] = -1
+    return fd, np.nonzero(~gone)[0].astype(np.int32)
+
+
+def test_oracle_deleted_faces_equal_packed_mesh():
+    # getTriangulation keeps {-1,-1,-1} for deleted faces (source/MRMesh/MRMeshTopology.cpp:430-441); the
+    # reference's tree covers the valid faces, keeping their ids (source/MRMesh/MRAABBTree.cpp:35-49):
+    # every result equals that of the packed mesh with the face ids mapped back
+    v, f = synth.bumpy_icosphere(3, 8, 5)
+    fd, ids = _delete_faces(f, 0.05, 3)
+    md, mp = O.Mesh(v, fd), O.Mesh(v, f[ids])
+    nd, npk = md.nodes(), mp.nodes()
+    assert len(nd) == 2 * len(ids) - 1
+    lf = npk["r"] < 0
+    npk["l"][lf] = ids[npk["l"][lf]]
+    assert np.array_equal(nd.view(np.uint8), npk.view(np.uint8))
+    assert np.array_equal(md.dipoles().view(np.uint8), mp.dipoles().view(np.uint8))
+    pts = synth.projection_queries(v, f, 4000, 12)
+    rd, rp = md.find_projections(pts), mp.find_projections(pts)
+    hit = rp["face"] >= 0
+    rp["face"][hit] = ids[rp["face"][hit]]
+    assert np.array_equal(rd.view(np.uint8), rp.view(np.uint8))
+    assert np.array_equal(md.fwn_vector(pts).view(np.uint32), mp.fwn_vector(pts).view(np.uint32))
+    dims, xf, _ = synth.grid_for_mesh(v, (12, 12, 12))
+    assert np.array_equal(md.sdf_grid(dims, xf).view(np.uint32), mp.sdf_grid(dims, xf).view(np.uint32))
+    sd, sp = md.self_intersections(), mp.self_intersections()
+    assert sd.shape[0] == f.shape[0] and not sd[fd[:, 0] < 0].any()
+    assert np.array_equal(sd[ids], sp)
+    # the brute-force checkers skip deleted faces too
+    assert np.array_equal(O.bf_nearest_f64(v, fd, pts[:300]), O.bf_nearest_f64(v, f[ids], pts[:300]))
+
+
+def test_product_host_tree_deleted_faces():
+    import meshlib_amd as M
+    v, f = synth.bumpy_icosphere(4, 8, 6)
+    fd, ids = _delete_faces(f, 0.05, 9)
+    nodes, dips, depth = M.build_tree_host(v, fd)
+    om = O.Mesh(v, fd)
+    assert len(nodes) == 2 * len(ids) - 1
+    assert np.array_equal(nodes.view(np.uint8), om.nodes().view(np.uint8))
+    assert np.array_equal(dips.view(np.uint8), om.dipoles().view(np.uint8))
+    # every face deleted: an empty tree
+    nodes, dips, depth = M.build_tree_host(v, np.full_like(f, -1))
+    assert len(nodes) == 0
 
 
 def test_product_host_tree_matches_oracle():
