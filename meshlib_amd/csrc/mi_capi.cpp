@@ -201,8 +201,16 @@ int ensureWorkspace(mi_mesh* m, size_t bytes)
     return MI_OK;
 }
 
+// bytes per lane of the signed-distance split form: the distance walk's float best, or the keyed walk's
+// (order value, key) for minDistSq > 0
+int sdfLaneBytes(const mi_dist_options* o) { return o && o->minDistSq > 0 ? 8 : 4; }
+
+// device bytes per voxel of a signed-distance chunk: the output, and the split form's scratch
+// (lane bytes + the cut-order byte + the unit list)
+size_t sdfBytesPerVoxel(const mi_dist_options* o) { return o ? sizeof(float) + sdfLaneBytes(o) + 2 : sizeof(float); }
+
 // scratch of the split form for grids of up to `layers` z-layers, in `buf` (grown if needed)
-int splitScratch(mi_mesh* m, DevBuf& buf, const int32_t dims[3], int32_t layers, mi::SplitScratch& sc)
+int splitScratch(mi_mesh* m, DevBuf& buf, const int32_t dims[3], int32_t layers, int laneBytes, mi::SplitScratch& sc)
 {
     sc = mi::SplitScratch{};
     if (layers <= 0) return MI_OK;
@@ -211,9 +219,9 @@ int splitScratch(mi_mesh* m, DevBuf& buf, const int32_t dims[3], int32_t layers,
     const int setting = g_splitBudget.load();
     const int budget = setting >= 0 ? setting : autoSdfBudget(m->depth, waves);
     if (budget <= 0) return MI_OK;
-    const size_t bytes = mi::splitScratchBytes(waves, 4);
+    const size_t bytes = mi::splitScratchBytes(waves, laneBytes);
     if (buf.bytes < bytes) MI_HIP(buf.alloc(bytes));
-    mi::splitScratchBind(buf.p, waves, 4, budget, sc);
+    mi::splitScratchBind(buf.p, waves, laneBytes, budget, sc);
     return MI_OK;
 }
 
@@ -596,7 +604,7 @@ int mi_sdf_grid_device(mi_mesh* m, const int32_t dims[3], const float xf[12], co
     mi::DistOpts o{ opts->minDistSq, opts->maxDistSq, opts->nullOutsideMinMax, opts->windingNumberThreshold,
                     opts->windingNumberBeta };
     mi::SplitScratch sc;
-    if (int rc = splitScratch(m, m->ws, dims, zEnd - zBegin, sc)) return rc;
+    if (int rc = splitScratch(m, m->ws, dims, zEnd - zBegin, sdfLaneBytes(opts), sc)) return rc;
     MI_HIP(mi::launchSdfGrid(m->dev, dims, toXf(xf), o, zBegin, zEnd, d_res, m->stackD, pick(m, stream), nullptr,
                              nullptr, nullptr, &sc));
     return MI_OK;
@@ -753,15 +761,14 @@ static int gridHost(mi_mesh* m, const int32_t dims[3], const float xf[12], const
     if (!report(cb, user, 0.5f)) return fail(MI_E_CANCELED, "Operation was canceled");
     if (total == 0) return MI_OK;
     // maxBufferSizeAlignedByBlock (source/MRCuda/MRCudaBasic.cpp:103-107): whole z-layers
-    // the split form of the signed-distance walk needs about as many scratch bytes as the output
-    const size_t perVoxel = opts ? 2 * sizeof(float) : sizeof(float);
+    const size_t perVoxel = sdfBytesPerVoxel(opts);
     const size_t layers = std::min(memBudget() / perVoxel / layer, (size_t)dims[2]);
     if (layers == 0) return fail(MI_E_LIMIT, "one z-layer does not fit in the device memory budget");
     DevBuf out, splitBuf;
     MI_HIP(out.alloc(layers * layer * sizeof(float)));
     mi::SplitScratch sc;
     if (opts)
-        if (int rc = splitScratch(m, splitBuf, dims, (int32_t)layers, sc)) return rc;
+        if (int rc = splitScratch(m, splitBuf, dims, (int32_t)layers, sdfLaneBytes(opts), sc)) return rc;
     if (!report(cb, user, 0.6f)) return fail(MI_E_CANCELED, "Operation was canceled");
     const int chunks = (int)((dims[2] + layers - 1) / layers);
     const mi::Xf x = toXf(xf);
@@ -812,7 +819,7 @@ static int gridByParts(mi_mesh* m, mi_grid_part_fn fn, void* fnUser, const int32
     if (!report(cb, user, 0.5f)) return fail(MI_E_CANCELED, "Operation was canceled");
     if (total == 0) return MI_OK;
     // two device buffers + two pinned host buffers, each of `chunkLayers` whole z-layers
-    const size_t perVoxel = opts ? 2 * sizeof(float) : sizeof(float);
+    const size_t perVoxel = sdfBytesPerVoxel(opts);
     const size_t chunkLayers = std::min(memBudget() / perVoxel / layer / 2, (size_t)dims[2]);
     if (chunkLayers == 0) return fail(MI_E_LIMIT, "one z-layer does not fit in the device memory budget");
     // splitByChunks (source/MRMesh/MRChunkIterator.cpp:29-53) in units of layers
@@ -827,7 +834,7 @@ static int gridByParts(mi_mesh* m, mi_grid_part_fn fn, void* fnUser, const int32
     mi::SplitScratch sc[2];
     if (opts)
         for (int i = 0; i < 2; ++i)
-            if (int rc = splitScratch(m, splitBuf[i], dims, (int32_t)chunkLayers, sc[i])) return rc;
+            if (int rc = splitScratch(m, splitBuf[i], dims, (int32_t)chunkLayers, sdfLaneBytes(opts), sc[i])) return rc;
     struct Host {
         float* hbuf[2] = { nullptr, nullptr };
         hipStream_t side = nullptr, main = nullptr;

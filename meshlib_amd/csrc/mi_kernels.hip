@@ -114,194 +114,6 @@ __global__ __launch_bounds__(kBlock) void kSelfIntersections(MeshDev m, int nLog
     out[L.face] = (wn < 0 || wn > 1) ? 1 : 0;
 }
 
-template <bool XF, int D>
-__global__ __launch_bounds__(kBlock) void kProjections(MeshDev m, const float* __restrict__ pts,
-                                                       const uint32_t* __restrict__ perm, int64_t n,
-                                                       int nLogical, Xf ptXf, int hasPtXf, Xf treeXf,
-                                                       float up, float lo, const float* __restrict__ bound,
-                                                       ProjRec* __restrict__ out)
-{
-    __shared__ int2 smem[D * kBlock];
-    const int lb = logicalBlock(nLogical);
-    if (lb >= nLogical) return;
-    const int64_t i = (int64_t)lb * kBlock + threadIdx.x;
-    if (i >= n) return;
-    const int64_t q = perm ? (int64_t)perm[i] : i;
-    float px = pts[3 * q], py = pts[3 * q + 1], pz = pts[3 * q + 2];
-    if (hasPtXf) xfApply(ptXf, px, py, pz);
-    float lim = up;
-    if (bound) { const float b = bound[q]; lim = b < up ? b : up; }
-    BestHit h;
-    h.px = h.py = h.pz = 0; h.a = h.b = 0;
-    distanceDfs<true, XF, D>(m, px, py, pz, lim, lo, treeXf, smem + threadIdx.x, h);
-    float4* o = reinterpret_cast<float4*>(out + q);
-    o[0] = make_float4(__int_as_float(h.face), h.px, h.py, h.pz);
-    o[1] = make_float4(__int_as_float(-1), h.a, h.b, h.distSq);
-}
-
-// bound (optional, indexed like the output): a per-query cap on upDistLimitSq.  With rerun lists
-// given, a query whose cap was below `up` and that found nothing below the cap is not written but
-// appended to the list, to be searched again without the cap (any cap above the true distance
-// gives the reference's record, so the combination is exact).
-template <int D>
-__global__ __launch_bounds__(kBlock) void kProjectionsCoop(MeshDev m, const float* __restrict__ pts,
-                                                           const uint32_t* __restrict__ perm, int64_t n,
-                                                           int nLogical, Xf ptXf, int hasPtXf, float up,
-                                                           const float* __restrict__ bound,
-                                                           uint32_t* __restrict__ rerun,
-                                                           uint32_t* __restrict__ rerunCount,
-                                                           SplitScratch sc, ProjRec* __restrict__ out)
-{
-    __shared__ ProjStack<D> stkMem[kWaves];
-    const int lb = logicalBlock(nLogical);
-    if (lb >= nLogical) return;
-    const int64_t i = (int64_t)lb * kBlock + threadIdx.x;
-    const bool valid = i < n;
-    if (!waveBallot(valid)) return;
-    const int64_t q = valid ? (perm ? (int64_t)perm[i] : i) : 0;
-    float px = 0, py = 0, pz = 0;
-    float lim = up;
-    if (valid) {
-        px = pts[3 * q]; py = pts[3 * q + 1]; pz = pts[3 * q + 2];
-        if (hasPtXf) xfApply(ptXf, px, py, pz);
-        if (bound) { const float b = bound[q]; lim = b < up ? b : up; }
-    }
-    const int w = threadIdx.x >> 6;
-    BestHit h;
-    uint32_t key = 0;
-    bool stopped = false;
-    coopProject<D>(m, px, py, pz, lim, valid, &stkMem[w], h, &key, sc.list ? sc.budget : 0, &stopped);
-    if (stopped) {
-        // hand the wave to the split pass with every lane's (distSq, key) so far
-        const int lane = threadIdx.x & 63;
-        uint32_t slot = 0;
-        if (lane == 0) slot = atomicAdd(&sc.counters[0], 1u);
-        slot = (uint32_t)uniform((int)slot);
-        if (lane == 0) sc.list[slot] = (uint32_t)(lb * kWaves + w);
-        reinterpret_cast<unsigned long long*>(sc.best)[(size_t)slot * 64 + lane] =
-            ((unsigned long long)__float_as_uint(h.distSq) << 32) | key;
-        rankCut(m, valid, px, py, pz, sc.order + (size_t)slot * 64);
-        return;
-    }
-    const bool again = rerun && valid && h.face < 0 && lim < up;
-    if (rerun) appendWave(again, (uint32_t)q, rerun, rerunCount);
-    if (!valid || again) return;
-    float4* o = reinterpret_cast<float4*>(out + q);
-    o[0] = make_float4(__int_as_float(h.face), h.px, h.py, h.pz);
-    o[1] = make_float4(__int_as_float(-1), h.a, h.b, h.distSq);
-}
-
-// Second launch of the split projections: persistent waves take (handed-over wave, cut subtree)
-// pairs; each lane's key prefix for the subtree is its near/far choices along the subtree's path,
-// the walk starts from the lane's (distSq, key) so far, and improvements are merged with an atomic
-// minimum on distSq bits << 32 | key -- the order (distSq, key) that decides the record.
-template <int D>
-__global__ __launch_bounds__(kBlock) void kProjSplit(MeshDev m, const float* __restrict__ pts,
-                                                       const uint32_t* __restrict__ perm, int64_t n, Xf ptXf,
-                                                       int hasPtXf, float up, const float* __restrict__ bound,
-                                                       SplitScratch sc)
-{
-    __shared__ ProjStack<D> stkMem[kWaves];
-    const int lane = threadIdx.x & 63;
-    const int w = threadIdx.x >> 6;
-    unsigned long long* best = reinterpret_cast<unsigned long long*>(sc.best);
-    const uint32_t nHard = sc.counters[0] < sc.cap ? sc.counters[0] : sc.cap;
-    for (;;) {
-        uint32_t t = 0;
-        if (lane == 0) t = atomicAdd(&sc.counters[1], 1u);
-        t = (uint32_t)uniform((int)t);
-        const uint32_t hIdx = t / (uint32_t)m.nCut, r = t % (uint32_t)m.nCut;
-        if (hIdx >= nHard) break;
-        const uint32_t c = sc.order[(size_t)hIdx * 64 + r];
-        if (c >= (uint32_t)m.nCut) continue;     // rankCut writes a permutation of [0, nCut)
-        int64_t q;
-        float px, py, pz, lim;
-        const bool valid = projQuery(pts, perm, n, ptXf, hasPtXf, up, bound, sc.list[hIdx], lane, q, px, py, pz, lim);
-        const unsigned long long seed = best[(size_t)hIdx * 64 + lane];
-        BestHit res;
-        res.distSq = valid ? __uint_as_float((uint32_t)(seed >> 32)) : -1.f;
-        res.face = -1;
-        res.px = res.py = res.pz = res.a = res.b = 0;
-        uint32_t bestKey = valid ? (uint32_t)seed : 0u;
-        const CutEntry E = loadCut(m, c);
-        const float cd = boxDistSq(E.mn[0], E.mn[1], E.mn[2], E.mx[0], E.mx[1], E.mx[2], px, py, pz);
-        if (!waveBallot(cd <= res.distSq)) continue;     // no lane can improve inside this subtree
-        // key prefix: this lane's near(0)/far(1) choice at every level of the path to the subtree
-        uint32_t key = 0;
-        int node = m.rootRef;
-        for (int l = 0; l < E.depth; ++l) {
-            const float4 n0 = cload(m.dnode, 4 * (size_t)node), n1 = cload(m.dnode, 4 * (size_t)node + 1);
-            const float4 n2 = cload(m.dnode, 4 * (size_t)node + 2), n3 = cload(m.dnode, 4 * (size_t)node + 3);
-            const float d1 = boxDistSq(n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, px, py, pz);
-            const float d2 = boxDistSq(n1.z, n1.w, n2.x, n2.y, n2.z, n2.w, px, py, pz);
-            const bool leftNear = d1 < d2;
-            const bool right = (E.path >> l) & 1u;
-            if (right == leftNear) key |= 1u << (31 - l);
-            node = right ? __float_as_int(n3.y) : __float_as_int(n3.x);
-        }
-        bool st;
-        coopProjectWalk<D>(m, px, py, pz, E.ref, E.depth, cd, key, &stkMem[w], res, bestKey, 0, st, false);
-        if (valid && res.face >= 0) {
-            const unsigned long long packed = ((unsigned long long)__float_as_uint(res.distSq) << 32) | bestKey;
-            if (packed < seed) atomicMin(&best[(size_t)hIdx * 64 + lane], packed);
-        }
-    }
-}
-
-// Third launch: the handed-over waves' records.  The winning (distSq, key) names one leaf: walk down
-// from the root taking, at each level, the child this lane saw as near (key bit 0) or far (1).
-template <int D>
-__global__ __launch_bounds__(kBlock) void kProjSplitFinish(MeshDev m, const float* __restrict__ pts,
-                                                           const uint32_t* __restrict__ perm, int64_t n, Xf ptXf,
-                                                           int hasPtXf, float up, const float* __restrict__ bound,
-                                                           SplitScratch sc, ProjRec* __restrict__ out)
-{
-    const int lane = threadIdx.x & 63;
-    const unsigned long long* best = reinterpret_cast<const unsigned long long*>(sc.best);
-    const uint32_t nHard = sc.counters[0] < sc.cap ? sc.counters[0] : sc.cap;
-    for (;;) {
-        uint32_t t = 0;
-        if (lane == 0) t = atomicAdd(&sc.counters[2], 1u);
-        t = (uint32_t)uniform((int)t);
-        if (t >= nHard) break;
-        int64_t q;
-        float px, py, pz, lim;
-        const bool valid = projQuery(pts, perm, n, ptXf, hasPtXf, up, bound, sc.list[t], lane, q, px, py, pz, lim);
-        if (!valid) continue;
-        const unsigned long long b = best[(size_t)t * 64 + lane];
-        const float d = __uint_as_float((uint32_t)(b >> 32));
-        const uint32_t key = (uint32_t)b;
-        float4* o = reinterpret_cast<float4*>(out + q);
-        if (!(d < lim)) {
-            // nothing below the limit: the reference's empty result
-            o[0] = make_float4(__int_as_float(-1), 0.f, 0.f, 0.f);
-            o[1] = make_float4(__int_as_float(-1), 0.f, 0.f, lim);
-            continue;
-        }
-        int node = m.rootRef;
-        for (int l = 0; node >= 0 && l < 32; ++l) {
-            const float4 n0 = cload(m.dnode, 4 * (size_t)node), n1 = cload(m.dnode, 4 * (size_t)node + 1);
-            const float4 n2 = cload(m.dnode, 4 * (size_t)node + 2), n3 = cload(m.dnode, 4 * (size_t)node + 3);
-            const float d1 = boxDistSq(n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, px, py, pz);
-            const float d2 = boxDistSq(n1.z, n1.w, n2.x, n2.y, n2.z, n2.w, px, py, pz);
-            const bool leftNear = d1 < d2;
-            const bool far = (key >> (31 - l)) & 1u;
-            node = (far == leftNear) ? __float_as_int(n3.y) : __float_as_int(n3.x);
-        }
-        if (node >= 0) continue;     // a key always ends at a leaf of a tree of depth <= 32
-        const Leaf L = loadLeaf(m.tri, ~node);
-        const double pd[3] = { px, py, pz };
-        const double ad[3] = { L.a[0], L.a[1], L.a[2] };
-        const double bd[3] = { L.b[0], L.b[1], L.b[2] };
-        const double cd[3] = { L.c[0], L.c[1], L.c[2] };
-        double pr[3], ba, bb;
-        closestPointD(pd, ad, bd, cd, pr, ba, bb);
-        const float qx = (float)pr[0], qy = (float)pr[1], qz = (float)pr[2];
-        o[0] = make_float4(__int_as_float(L.face), qx, qy, qz);
-        o[1] = make_float4(__int_as_float(-1), (float)ba, (float)bb, d);
-    }
-}
-
 // At most 72 VGPRs: seven waves per SIMD (the LDS of seven blocks fits a CU); the walks are bound by
 // the latency of their dependent scalar loads, and the seventh wave hides more of it than the
 // registers it costs (signed distance 105 -> 98 ms at k=8 256^3).
@@ -859,8 +671,10 @@ hipError_t launchSdfGrid(const MeshDev& m, const int32_t dims[3], const Xf& xf, 
     if (bricks <= 0) return hipSuccess;
     const int nLogical = (int)((bricks + 3) / 4);
     // the wave-cooperative distance returns the minimum over candidates; with minDistSq > 0 the
-    // reference stops at the first candidate <= minDistSq in its own visit order, so that case
-    // keeps the per-lane traversal
+    // reference stops at the first candidate <= minDistSq in its own visit order, which the keyed walk
+    // reproduces (launchSdfGridKeyed; its split scratch holds 8 bytes per lane)
+    if (o.minDistSq > 0 && !bound && traversal() == Traversal::WaveCoop)
+        return launchSdfGridKeyed(m, dims, xf, o, zBegin, zEnd, out, stackD, s, split);
     const bool coop = (traversal() == Traversal::WaveCoop || bound) && !(o.minDistSq > 0);
     if (bound && !coop) return hipErrorInvalidValue;
     // The leaf filter saves instructions but adds a dependent load to every leaf visit.  It pays unless
@@ -966,50 +780,6 @@ hipError_t launchSelfIntersections(const MeshDev& m, float beta, uint8_t* out, i
         else
             hipLaunchKernelGGL(kSelfIntersections<D>, dim3(paddedGrid(nLogical)), dim3(kBlock), 0, s, m,
                                nLogical, beta, out);
-        return hipGetLastError();
-    });
-}
-
-hipError_t launchProjections(const MeshDev& m, const float* pts, const uint32_t* perm, int64_t n,
-                             const Xf* ptXf, const Xf* treeXf, float up, float lo, ProjRec* out,
-                             int stackD, hipStream_t s, const float* bound, uint32_t* rerun,
-                             uint32_t* rerunCount, const SplitScratch* split)
-{
-    if (n <= 0) return hipSuccess;
-    const int nLogical = (int)((n + kBlock - 1) / kBlock);
-    Xf px{}, tx{};
-    if (ptXf) px = *ptXf;
-    if (treeXf) tx = *treeXf;
-    // the shared walk reproduces the reference's record exactly when its early exit cannot pick a
-    // non-minimal leaf (loDistLimitSq <= 0) and the tree is not transformed per box
-    const bool coopOk = !(lo > 0) && !treeXf;
-    const bool coop = coopOk && (traversal() == Traversal::WaveCoop || rerun);
-    if (rerun && !coop) return hipErrorInvalidValue;
-    SplitScratch sc;
-    if (coop && split && split->list && split->budget > 0 && !rerun && m.nCut > 0) {
-        if ((int64_t)split->cap < projectionWaves(n)) return hipErrorInvalidValue;
-        sc = *split;
-        hipError_t e = fillU32(sc.counters, 3, 0u, s);
-        if (e != hipSuccess) return e;
-    }
-    return dispatchDepth(stackD, [&](auto Dc) {
-        constexpr int D = decltype(Dc)::value;
-        if (coop) {
-            hipLaunchKernelGGL(kProjectionsCoop<D>, dim3(paddedGrid(nLogical)), dim3(kBlock), 0, s, m, pts,
-                               perm, n, nLogical, px, ptXf ? 1 : 0, up, bound, rerun, rerunCount, sc, out);
-            if (sc.list) {
-                const int blocks = persistentBlocks();
-                hipLaunchKernelGGL(kProjSplit<D>, dim3(blocks), dim3(kBlock), 0, s, m, pts, perm, n, px,
-                                   ptXf ? 1 : 0, up, bound, sc);
-                hipLaunchKernelGGL(kProjSplitFinish<D>, dim3(blocks), dim3(kBlock), 0, s, m, pts, perm, n, px,
-                                   ptXf ? 1 : 0, up, bound, sc, out);
-            }
-        } else if (treeXf)
-            hipLaunchKernelGGL((kProjections<true, D>), dim3(paddedGrid(nLogical)), dim3(kBlock), 0, s,
-                               m, pts, perm, n, nLogical, px, ptXf ? 1 : 0, tx, up, lo, bound, out);
-        else
-            hipLaunchKernelGGL((kProjections<false, D>), dim3(paddedGrid(nLogical)), dim3(kBlock), 0, s,
-                               m, pts, perm, n, nLogical, px, ptXf ? 1 : 0, tx, up, lo, bound, out);
         return hipGetLastError();
     });
 }

@@ -124,6 +124,11 @@ hipError_t launchSdfGrid(const MeshDev& m, const int32_t dims[3], const Xf& xf, 
                          int32_t zBegin, int32_t zEnd, float* out, int stackD, hipStream_t s,
                          const float* bound = nullptr, uint32_t* rerun = nullptr, uint32_t* rerunCount = nullptr,
                          const SplitScratch* split = nullptr);
+// the same with minDistSq > 0 on the keyed walks (the reference's early exit at minDistSq reproduced
+// exactly); split scratch (optional) bound with 8 bytes per lane
+hipError_t launchSdfGridKeyed(const MeshDev& m, const int32_t dims[3], const Xf& xf, const DistOpts& o,
+                              int32_t zBegin, int32_t zEnd, float* out, int stackD, hipStream_t s,
+                              const SplitScratch* split);
 // the listed slab voxels, searched without a cap
 hipError_t launchSdfVoxels(const MeshDev& m, const int32_t dims[3], const Xf& xf, const DistOpts& o,
                            int32_t zBegin, const uint32_t* list, int64_t count, float* out, int stackD,

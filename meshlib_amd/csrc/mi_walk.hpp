@@ -747,19 +747,36 @@ struct ProjStack {
     int32_t win[64];                  // leaf of the lane's best found in this walk, -1 before
 };
 
-// Runs the `qn` queued projection leaf tests, one per lane, and refreshes this lane's (bestD, bestKey).
+// The order of candidates under loDistLimitSq (findProjectionSubtree, source/MRMesh/MRMeshProject.cpp:78-85):
+// the reference accepts a leaf on a strict '<' and stops at the first accepted one at or below lo.  With
+// loE = min(lo, the largest float below the search limit), every accepted leaf at or below lo is the
+// first such leaf of the lane's depth-first order, so the result is the leaf minimising
+// (max(d, loE), key): all candidates <= loE tie and the lowest key (the first visited) wins, and
+// without such a candidate the order is (d, key) as before.  loE <= 0 leaves d unchanged.  A NaN d
+// stays NaN (never accepted, as in the reference).
+__device__ __forceinline__ float effectiveLo(float lo, float lim)
+{
+    if (!(lo > 0) || !(lim > 0)) return 0.f;
+    const float below = __uint_as_float(__float_as_uint(lim) - 1u);     // largest float < lim (lim > 0)
+    return lo < below ? lo : below;
+}
+__device__ __forceinline__ float loOrder(float d, float loE) { return d < loE ? loE : d; }
+
+// Runs the `qn` queued projection leaf tests, one per lane, and refreshes this lane's (bestD, bestKey);
+// candidates compare by (loOrder(d, owner's loE), key), see effectiveLo.
 template <int D>
-__device__ __forceinline__ void flushProjLeaves(const MeshDev& m, float px, float py, float pz, ProjStack<D>* stk,
-                                                int qn, float& bestD, uint32_t& bestKey)
+__device__ __forceinline__ void flushProjLeaves(const MeshDev& m, float px, float py, float pz, float loE,
+                                                ProjStack<D>* stk, int qn, float& bestD, uint32_t& bestKey)
 {
     const int lane = threadIdx.x & 63;
     const int owner = stk->owner[lane];
     const float ox = __shfl(px, owner), oy = __shfl(py, owner), oz = __shfl(pz, owner);
+    const float oLo = __shfl(loE, owner);
     unsigned long long packed = ~0ull;
     int k = 0;
     if (lane < qn) {
         k = stk->leaf[lane];
-        const float d = leafDistSq(loadLeafLanes(m.tri, k), ox, oy, oz);
+        const float d = loOrder(leafDistSq(loadLeafLanes(m.tri, k), ox, oy, oz), oLo);
         packed = ((unsigned long long)__float_as_uint(d) << 32) | stk->key[lane];
         atomicMin(&stk->best[owner], packed);
     }
@@ -779,8 +796,10 @@ __device__ __forceinline__ float pruneLimit(float best) { return best == INFINIT
 
 // record = false leaves res's point, face and barycentrics unset (res.face >= 0 still marks an
 // improvement): for callers that keep only (distSq, key).
+// loE (per lane, see effectiveLo): res.distSq and the merged bests are in loOrder space; with record the
+// winning leaf's record (and its true squared distance) is recomputed into res.
 template <int D>
-__device__ __forceinline__ void coopProjectWalk(const MeshDev& m, float px, float py, float pz, int cur,
+__device__ __forceinline__ void coopProjectWalk(const MeshDev& m, float px, float py, float pz, float loE, int cur,
                                                 int curDepth, float curd, uint32_t curKey, ProjStack<D>* stk,
                                                 BestHit& res, uint32_t& bestKey, int budget, bool& stopped,
                                                 bool record = true)
@@ -827,7 +846,7 @@ __device__ __forceinline__ void coopProjectWalk(const MeshDev& m, float px, floa
         }
         if (cur < 0) {
             if (qn + __popcll(act) > 64) {
-                flushProjLeaves<D>(m, px, py, pz, stk, qn, res.distSq, bestKey);
+                flushProjLeaves<D>(m, px, py, pz, loE, stk, qn, res.distSq, bestKey);
                 bestT = pruneLimit(res.distSq);
                 qn = 0;
                 act = leMask(curd, bestT);
@@ -919,7 +938,7 @@ __device__ __forceinline__ void coopProjectWalk(const MeshDev& m, float px, floa
         sp += cnt - 1;
         have = true;
     }
-    if (qn) flushProjLeaves<D>(m, px, py, pz, stk, qn, res.distSq, bestKey);
+    if (qn) flushProjLeaves<D>(m, px, py, pz, loE, stk, qn, res.distSq, bestKey);
     // the record of the winning leaf (its distance is the one just merged, recomputed identically)
     const int wl = stk->win[lane];
     if (wl >= 0) {
@@ -934,16 +953,23 @@ __device__ __forceinline__ void coopProjectWalk(const MeshDev& m, float px, floa
             res.face = L.face;
             res.px = (float)pr[0]; res.py = (float)pr[1]; res.pz = (float)pr[2];
             res.a = (float)ba; res.b = (float)bb;
+            // the leaf test's value; a stopped walk keeps the loOrder value its hand-over merges with
+            if (!stopped) {
+                const float dx = res.px - px, dy = res.py - py, dz = res.pz - pz;
+                res.distSq = dx * dx + dy * dy + dz * dz;
+            }
         } else {
             res.face = 0;
         }
     }
 }
 
+// lo: the caller's loDistLimitSq (<= 0: none); the seed, pruning and hand-over bests are in loOrder
+// space (see effectiveLo), the record's distSq is the winner's true squared distance.
 template <int D>
 __device__ __forceinline__ void coopProject(const MeshDev& m, float px, float py, float pz, float up, bool valid,
                                             ProjStack<D>* stk, BestHit& res, uint32_t* bestKeyOut = nullptr,
-                                            int budget = 0, bool* stopped = nullptr)
+                                            int budget = 0, bool* stopped = nullptr, float lo = 0.f)
 {
     res.distSq = valid ? up : -1.f;
     res.face = -1;
@@ -955,8 +981,9 @@ __device__ __forceinline__ void coopProject(const MeshDev& m, float px, float py
     if (m.numFaces <= 0) return;
     // search radius just above the greedy leaf's distance: that leaf (and every leaf at the minimum)
     // is still accepted, so the record is unchanged; an empty search keeps the caller's limit
+    const float loE = effectiveLo(lo, up);
     if (valid && up > 0) {
-        const float g = greedyLeafDistSq(m, px, py, pz);
+        const float g = loOrder(greedyLeafDistSq(m, px, py, pz), loE);
         if (g < up) {
             // any radius above g keeps that leaf; at least FLT_MIN so no comparison meets a denormal
             const float above = g < FLT_MIN ? FLT_MIN : __uint_as_float(__float_as_uint(g) + 1u);
@@ -966,7 +993,7 @@ __device__ __forceinline__ void coopProject(const MeshDev& m, float px, float py
     const float rootd = boxDistSq(m.rootMin[0], m.rootMin[1], m.rootMin[2], m.rootMax[0], m.rootMax[1],
                                   m.rootMax[2], px, py, pz);
     bool st = false;
-    coopProjectWalk<D>(m, px, py, pz, m.rootRef, 0, rootd, 0u, stk, res, bestKey, budget, st);
+    coopProjectWalk<D>(m, px, py, pz, loE, m.rootRef, 0, rootd, 0u, stk, res, bestKey, budget, st);
     if (stopped) *stopped = st;
     if (bestKeyOut) *bestKeyOut = bestKey;
 }

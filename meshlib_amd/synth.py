@@ -135,3 +135,13 @@ def projection_queries(verts: np.ndarray, tris: np.ndarray, n: int, seed: int,
     ps += rng.normal(0.0, sigma, size=(n_s, 3))
     pts = np.concatenate([pu, ps], axis=0).astype(np.float32)
     return np.ascontiguousarray(pts)
+
+
+def batched_workload_specs(items: int = 64, seed: int = 4):
+    """The uneven batched workload: per item (split level, bumps, mesh seed, grid side, query count,
+    query seed) -- levels uniform in 4..8, grid sides from {32, 64, 96, 128, 192}, 2^18..2^21 queries."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    levels = rng.integers(4, 9, size=items)
+    sides = rng.choice([32, 64, 96, 128, 192], size=items)
+    nqs = rng.integers(1 << 18, (1 << 21) + 1, size=items)
+    return [(int(levels[i]), 32, 1000 + i, int(sides[i]), int(nqs[i]), 2000 + i) for i in range(items)]

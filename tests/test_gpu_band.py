@@ -1,0 +1,149 @@
+"""Band options on the wave-cooperative and split walks, against the oracle.
+
+* findProjections with loDistLimitSq > 0: the reference stops at the first leaf of its depth-first
+  order at or below lo (source/MRMesh/MRMeshProject.cpp:78-85); the keyed walks reproduce that record
+  bit-for-bit for every split budget, the per-lane (reference-form) kernel as well.
+* calcFromGridWithDistances with minDistSq > 0, nullOutsideMinMax true and false: the option shape
+  of mcOffsetMesh (source/MRVoxels/MROffset.cpp:176-183), also through
+  calcFromGridWithDistancesByParts with layerOverlap 1 (MROffset.cpp:217).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import meshlib_amd as M
+from meshlib_amd import synth
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+F32MAX = float(np.finfo(np.float32).max)
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture
+def restore_modes():
+    old = (M.get_split_budget("sdf"), M.get_split_budget("projections"), M.get_traversal())
+    yield
+    M.set_split_budget(old[0], "sdf")
+    M.set_split_budget(old[1], "projections")
+    M.set_traversal(old[2])
+
+
+def offset_options(vox: float, offset: float, null_outside: bool):
+    # DistanceToMeshOptions of mcOffsetMesh (source/MRVoxels/MROffset.cpp:176-183)
+    a = abs(offset)
+    o = M.DistanceToMeshOptions()
+    o.minDistSq = float(np.float32(max(a - 1.001 * vox, 0.0)) ** 2)
+    o.maxDistSq = float(np.float32(a + 1.001 * vox) ** 2)
+    o.nullOutsideMinMax = null_outside
+    return o
+
+
+def oracle_opts(o):
+    return O.default_opts(beta=o.windingNumberBeta, minDistSq=o.minDistSq, maxDistSq=o.maxDistSq,
+                          nullOutsideMinMax=o.nullOutsideMinMax, threshold=o.windingNumberThreshold)
+
+
+@pytest.mark.parametrize("level,seed", [(5, 2), (8, 4)])
+def test_projections_lo_limit(level, seed, restore_modes):
+    v, f = synth.bumpy_icosphere(level, 32 if level == 8 else 8, seed)
+    gm, om = M.Mesh(v, f), O.Mesh(v, f)
+    pr = M.PointsToMeshProjector()
+    pr.updateMeshData(gm)
+    n = 40000 if level == 8 else 20000
+    pts = synth.projection_queries(v, f, n, seed + 30)
+    diag = float(np.linalg.norm(v.max(0) - v.min(0)))
+    cases = [(F32MAX, (0.01 * diag) ** 2), (F32MAX, (0.05 * diag) ** 2), ((0.04 * diag) ** 2, (0.02 * diag) ** 2),
+             ((0.03 * diag) ** 2, (0.05 * diag) ** 2)]
+    for up, lo in cases:
+        ref = om.find_projections(pts, up=up, lo=lo)
+        hits = int(np.count_nonzero((ref["face"] >= 0) & (ref["distSq"] <= lo)))
+        say(f"k={level} up {up:.3g} lo {lo:.3g}: {hits} early exits")
+        assert hits > n // 50
+        for mode, budget in (("wave_coop", -1), ("wave_coop", 1), ("wave_coop", 16), ("wave_coop", 0),
+                             ("per_lane", -1)):
+            M.set_traversal(mode)
+            M.set_split_budget(budget, "projections")
+            got = pr.findProjections(pts, upDistLimitSq=up, loDistLimitSq=lo)
+            assert np.array_equal(got.view(np.uint8), ref.view(np.uint8)), (mode, budget, up, lo)
+
+
+def test_projections_lo_bounded_device(restore_modes):
+    # per-point upper limits (ICP's pattern) together with lo: lo' = min(lo, below(limit)) per point
+    import torch
+    v, f = synth.bumpy_icosphere(6, 16, 9)
+    gm, om = M.Mesh(v, f), O.Mesh(v, f)
+    pr = M.PointsToMeshProjector()
+    pr.updateMeshData(gm)
+    n = 30000
+    pts = synth.projection_queries(v, f, n, 41)
+    diag = float(np.linalg.norm(v.max(0) - v.min(0)))
+    rng = np.random.default_rng(3)
+    ups = (rng.uniform(0.0, 0.06, n) * diag) ** 2
+    ups[::7] = F32MAX
+    ups = ups.astype(np.float32)
+    lo = float((0.02 * diag) ** 2)
+    ref = om.find_projections_bounded(pts, ups, lo=lo)
+    dp = torch.from_numpy(pts).cuda()
+    du = torch.from_numpy(ups).cuda()
+    dr = torch.zeros(n * 8, dtype=torch.int32, device="cuda")
+    for budget in (-1, 1):
+        M.set_split_budget(budget, "projections")
+        pr.find_projections_bounded_device(dp.data_ptr(), n, du.data_ptr(), dr.data_ptr(), loDistLimitSq=lo)
+        torch.cuda.synchronize()
+        assert np.array_equal(dr.cpu().numpy().view(np.uint8), ref.view(np.uint8)), budget
+
+
+@pytest.mark.parametrize("null_outside", [False, True])
+def test_sdf_band_config1(null_outside, restore_modes):
+    # config-1 size: k=5 mesh, 64^3 grid, the offset caller's band around a 3-voxel offset
+    v, f = synth.bumpy_icosphere(5, 8, 1)
+    dims, xf, vox = synth.grid_for_mesh(v, (64, 64, 64))
+    gm, om = M.Mesh(v, f), O.Mesh(v, f)
+    fwn = M.FastWindingNumber(gm)
+    vs = float(np.max(vox))
+    for offset in (3.0 * vs, -2.0 * vs, 0.5 * vs):
+        o = offset_options(vs, offset, null_outside)
+        ref = om.sdf_grid(dims, xf, oracle_opts(o))
+        early = np.count_nonzero(~np.isnan(ref) & (ref.astype(np.float64) ** 2 <= o.minDistSq)) if o.minDistSq > 0 else 0
+        say(f"offset {offset / vs:+.1f} vox, null {null_outside}: {np.isnan(ref).mean():.3f} NaN, {early} at/below min")
+        for budget in (-1, 1, 0):
+            M.set_traversal("wave_coop")
+            M.set_split_budget(budget, "sdf")
+            got = fwn.calcFromGridWithDistances(dims, xf, o)
+            assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), (offset, budget)
+        M.set_traversal("per_lane")
+        got = fwn.calcFromGridWithDistances(dims, xf, o)
+        assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), (offset, "per_lane")
+
+
+def test_sdf_band_by_parts_offset_shape(restore_modes):
+    # calcFromGridWithDistancesByParts with layerOverlap 1 and the mcOffsetMesh options, forced into
+    # several parts; the parts reassemble to the oracle's volume
+    v, f = synth.bumpy_icosphere(6, 16, 7)
+    dims, xf, vox = synth.grid_for_mesh(v, (72, 80, 96))
+    gm, om = M.Mesh(v, f), O.Mesh(v, f)
+    fwn = M.FastWindingNumber(gm)
+    vs = float(np.max(vox))
+    o = offset_options(vs, 2.5 * vs, False)
+    ref = om.sdf_grid(dims, xf, oracle_opts(o))
+    layer = int(dims[0]) * int(dims[1])
+    M.set_memory_limit(layer * 14 * 40)
+    try:
+        got = np.full(ref.size, -9.0, np.float32)
+        zs = []
+
+        def take(data, pdims, z0):
+            zs.append((z0, int(pdims[2])))
+            got[z0 * layer:(z0 + int(pdims[2])) * layer] = data
+
+        fwn.calcFromGridWithDistancesByParts(take, dims, xf, o, 1)
+    finally:
+        M.set_memory_limit(0)
+    assert len(zs) > 2 and zs[1][0] == zs[0][0] + zs[0][1] - 1      # one layer of overlap
+    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))

@@ -198,7 +198,7 @@ def test_chunked_and_by_parts(k5):
     pfull = pr.findProjections(pts)
     wfull = fwn.calcFromVector(pts, 2.0)
     try:
-        M.set_memory_limit(32 * 24 * 4 * 7)                 # 7 z-layers per chunk
+        M.set_memory_limit(32 * 24 * 4 * 7)                 # several z-layers per chunk
         say("chunked grid")
         assert np.array_equal(_bits(fwn.calcFromGridWithDistances(dims, xf, M.DistanceToMeshOptions())),
                               _bits(full))
@@ -206,7 +206,7 @@ def test_chunked_and_by_parts(k5):
         say("chunked points")
         assert np.array_equal(_bits(pr.findProjections(pts)), _bits(pfull))
         assert np.array_equal(_bits(fwn.calcFromVector(pts, 2.0)), _bits(wfull))
-        M.set_memory_limit(32 * 24 * 4 * 2 * 6)             # 6 layers per buffer, 2 buffers
+        M.set_memory_limit(32 * 24 * 10 * 2 * 3)            # 3 layers per buffer (10 B per voxel), 2 buffers
         for overlap in (0, 1, 2):
             parts = []
             say("by parts overlap", overlap)
