@@ -149,6 +149,28 @@ int  mi_mesh_create(const float* verts, int32_t nv, const int32_t* tris, int32_t
 void mi_mesh_destroy(mi_mesh* m);
 /* tree nodes: 2 * (valid faces) - 1, or 0 */
 int  mi_mesh_num_nodes(const mi_mesh* m);
+
+/* Measurement hook (no counterpart in the reference): while d_counters (device, zeroed by the
+ * caller, MI_STAT_LAUNCHES * MI_STAT_STRIDE uint64) is set, the wave-cooperative walks of every
+ * later call on this mesh add the records they read: block k counts a call's launch k (0 the first
+ * launch, 1 the split form's partial walks, 2 its last launch), slot MI_STAT_* within the block.
+ * Counts are per wave (a record read once for 64 lanes counts once).  NULL turns counting off.
+ * Counting costs a few scalar additions per step and one atomic per walk; results are unchanged. */
+#define MI_STAT_WALKS 0          /* walks run */
+#define MI_STAT_QUAD_STEPS 1     /* distance walk: 4-wide node visits (128-B quad record) */
+#define MI_STAT_PRISM_READS 2    /* distance walk: leaf visits reading the plane record (80 B) */
+#define MI_STAT_LEAF_TESTS 3     /* distance walk: leaf tests (48-B triangle record each) */
+#define MI_STAT_KEYED_STEPS 4    /* keyed (projection) walk: node visits (128-B quad + 48-B node record) */
+#define MI_STAT_KEYED_TESTS 5    /* keyed walk: leaf tests (48-B triangle record each) */
+#define MI_STAT_FWN_VISITS 6     /* winding number: node visits (32-B dipole record) */
+#define MI_STAT_FWN_DESCENTS 7   /* winding number: descents into children (16-B child refs) */
+#define MI_STAT_FWN_LEAVES 8     /* winding number: exact solid angles (48-B triangle record) */
+#define MI_STAT_SEEDS 9          /* greedy descents seeding a walk (one root-to-leaf path per wave) */
+#define MI_STAT_SEED_LEVELS 10   /* their levels (one 64-B node record each) */
+#define MI_STAT_COUNT 11
+#define MI_STAT_STRIDE 16
+#define MI_STAT_LAUNCHES 3
+int  mi_mesh_set_stats(mi_mesh* m, uint64_t* d_counters);
 /* exports the tree / dipoles in the reference layout (AABBTree::nodes(), Mesh::getDipoles()) */
 int  mi_mesh_get_tree(const mi_mesh* m, mi_tree_node* nodes, mi_dipole* dipoles);
 /* depth of the tree (longest root-to-leaf path, in edges) */

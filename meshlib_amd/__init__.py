@@ -39,6 +39,11 @@ DIPOLE_DTYPE = np.dtype([("pos", "<f4", 3), ("area", "<f4"), ("dirArea", "<f4", 
 
 MI_OK, MI_E_INVALID, MI_E_HIP, MI_E_CANCELED, MI_E_NOMEM, MI_E_LIMIT = 0, -1, -2, -3, -4, -5
 
+# measurement counters of mi_mesh_set_stats: names of the slots and the bytes each event reads
+STAT_NAMES = ("walks", "quad_steps", "prism_reads", "leaf_tests", "keyed_steps", "keyed_tests", "fwn_visits",
+              "fwn_descents", "fwn_leaves", "seeds", "seed_levels")
+STAT_STRIDE, STAT_LAUNCHES = 16, 3
+
 
 class MeshLibAmdError(RuntimeError):
     def __init__(self, code: int, msg: str):
@@ -105,6 +110,7 @@ def lib():
         "mi_mesh_num_nodes": (C.c_int, [P]),
         "mi_mesh_get_tree": (C.c_int, [P, P, P]),
         "mi_mesh_depth": (C.c_int, [P]),
+        "mi_mesh_set_stats": (C.c_int, [P, P]),
         "mi_build_tree_host": (C.c_int, [P, I32, P, I32, P, P, P]),
         "mi_fwn_calc_from_vector": (C.c_int, [P, P, I64, F, I32, P, _PROGRESS, P]),
         "mi_fwn_calc_self_intersections": (C.c_int, [P, F, P, _PROGRESS, P]),
@@ -318,6 +324,11 @@ class Mesh:
 
     def depth(self) -> int:
         return int(lib().mi_mesh_depth(self._h))
+
+    def set_stats(self, d_counters: int):
+        """Measurement hook: device pointer to MI_STAT_LAUNCHES * MI_STAT_STRIDE zeroed uint64 counters
+        that later calls on this mesh add their walks' record reads to (0 = off)."""
+        _check(lib().mi_mesh_set_stats(self._h, C.c_void_p(d_counters or None)))
 
 
 class _BatchItem(C.Structure):

@@ -546,6 +546,16 @@ void mi_mesh_destroy(mi_mesh* m)
 
 int mi_mesh_num_nodes(const mi_mesh* m) { return m ? m->numNodes : 0; }
 
+int mi_mesh_set_stats(mi_mesh* m, uint64_t* d_counters)
+{
+    if (!m) return fail(MI_E_INVALID, "mesh is null");
+    static_assert(MI_STAT_STRIDE == mi::kStatStride && MI_STAT_LAUNCHES == mi::kStatLaunches, "counter layout");
+    static_assert(MI_STAT_COUNT == mi::kStatCount, "counter layout");
+    std::lock_guard<std::mutex> lk(m->mu);
+    m->dev.stats = reinterpret_cast<unsigned long long*>(d_counters);
+    return MI_OK;
+}
+
 int mi_mesh_depth(const mi_mesh* m) { return m ? m->depth : 0; }
 
 int mi_build_tree_host(const float* verts, int32_t nv, const int32_t* tris, int32_t nf, mi_tree_node* nodes,

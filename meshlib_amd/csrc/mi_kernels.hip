@@ -701,9 +701,9 @@ hipError_t launchSdfGrid(const MeshDev& m, const int32_t dims[3], const Xf& xf, 
                                zBegin, zEnd, nbx, nby, nbz, nLogical, xf, o, bound, rerun, rerunCount, sc, out);
             if (sc.list) {
                 const int blocks = persistentBlocks();
-                hipLaunchKernelGGL(splitKernel, dim3(blocks), dim3(kBlock), 0, s, m, dims[0], dims[1], zBegin, zEnd,
+                hipLaunchKernelGGL(splitKernel, dim3(blocks), dim3(kBlock), 0, s, statLaunch(m, 1), dims[0], dims[1], zBegin, zEnd,
                                    nbx, nby, nbz, xf, sc);
-                hipLaunchKernelGGL(kSdfSplitFinish<D>, dim3(blocks), dim3(kBlock), 0, s, m, dims[0], dims[1],
+                hipLaunchKernelGGL(kSdfSplitFinish<D>, dim3(blocks), dim3(kBlock), 0, s, statLaunch(m, 2), dims[0], dims[1],
                                    zBegin, zEnd, nbx, nby, nbz, xf, o, sc, out);
             }
         } else {

@@ -18,6 +18,27 @@ struct CutEntry {
 constexpr int kCutDepth = 6;
 constexpr int kMaxCut = 1 << kCutDepth;
 
+// Measurement hook: per-launch counters of the records the wave-cooperative walks read, added by
+// lane 0 of each wave at the end of a walk when MeshDev::stats is set.  A call's launches use
+// consecutive blocks of kStatStride counters (its first launch block 0, a split pass block 1, the
+// split form's last launch block 2).
+enum StatSlot : int {
+    kStatWalks = 0,       // walks run (one per wave and walk)
+    kStatQuadSteps,       // distance walk: 4-wide node visits (128-B quad record each)
+    kStatPrismReads,      // distance walk: leaf visits reading the plane record (80 B each)
+    kStatLeafTests,       // distance walk: leaf tests run (48-B triangle record each)
+    kStatKeyedSteps,      // keyed walk: node visits (128-B quad + 48-B node record each)
+    kStatKeyedTests,      // keyed walk: leaf tests run (48-B triangle record each)
+    kStatFwnVisits,       // winding number: node visits (32-B dipole record each)
+    kStatFwnDescents,     // winding number: descents into an internal node's children (16-B refs)
+    kStatFwnLeaves,       // winding number: exact solid angles (48-B triangle record each)
+    kStatSeeds,           // greedy descents seeding a walk (one root-to-leaf path per wave)
+    kStatSeedLevels,      // their levels (one 64-B node record each)
+    kStatCount
+};
+constexpr int kStatStride = 16;
+constexpr int kStatLaunches = 3;
+
 struct MeshDev {
     const float4* dnode;   // 4 per internal node
     const float4* tri;     // 3 per leaf slot
@@ -31,7 +52,16 @@ struct MeshDev {
     int32_t rootRef;
     int32_t numFaces;
     int32_t depth;         // levels of the tree
+    unsigned long long* stats;   // measurement counters (kStatStride per launch) or null
 };
+
+// the mesh view of a call's launch number `launch` (its own block of counters)
+inline MeshDev statLaunch(const MeshDev& m, int launch)
+{
+    MeshDev r = m;
+    if (r.stats) r.stats += (size_t)launch * kStatStride;
+    return r;
+}
 
 struct Xf { float m[12]; };   // row-major A then b (AffineXf3f)
 

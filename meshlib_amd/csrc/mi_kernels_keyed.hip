@@ -346,8 +346,10 @@ hipError_t launchProjections(const MeshDev& m, const float* pts, const uint32_t*
             if (sc.list) {
                 const PointSource src{ pts, perm, n, px, ptXf ? 1 : 0, up, lo, bound };
                 const int blocks = persistentBlocks();
-                hipLaunchKernelGGL((kKeyedSplit<D, PointSource>), dim3(blocks), dim3(kBlock), 0, s, m, src, sc);
-                hipLaunchKernelGGL(kProjSplitFinish<D>, dim3(blocks), dim3(kBlock), 0, s, m, src, sc, out);
+                hipLaunchKernelGGL((kKeyedSplit<D, PointSource>), dim3(blocks), dim3(kBlock), 0, s, statLaunch(m, 1),
+                                   src, sc);
+                hipLaunchKernelGGL(kProjSplitFinish<D>, dim3(blocks), dim3(kBlock), 0, s, statLaunch(m, 2), src, sc,
+                                   out);
             }
         } else if (treeXf)
             hipLaunchKernelGGL((kProjections<true, D>), dim3(paddedGrid(nLogical)), dim3(kBlock), 0, s,
@@ -381,8 +383,9 @@ hipError_t launchSdfGridKeyed(const MeshDev& m, const int32_t dims[3], const Xf&
                            out);
         if (sc.list) {
             const int blocks = persistentBlocks();
-            hipLaunchKernelGGL((kKeyedSplit<D, BrickSource>), dim3(blocks), dim3(kBlock), 0, s, m, src, sc);
-            hipLaunchKernelGGL(kSdfBandFinish<D>, dim3(blocks), dim3(kBlock), 0, s, m, src, o, sc, out);
+            hipLaunchKernelGGL((kKeyedSplit<D, BrickSource>), dim3(blocks), dim3(kBlock), 0, s, statLaunch(m, 1), src,
+                               sc);
+            hipLaunchKernelGGL(kSdfBandFinish<D>, dim3(blocks), dim3(kBlock), 0, s, statLaunch(m, 2), src, o, sc, out);
         }
         return hipGetLastError();
     });

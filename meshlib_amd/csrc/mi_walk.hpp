@@ -395,6 +395,12 @@ __device__ __forceinline__ float laneValue(float v, int lane)
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
+// adds a wave-uniform count to the mesh's measurement counter `slot` (lane 0; nothing without counters)
+__device__ __forceinline__ void statAdd(const MeshDev& m, int slot, uint32_t v)
+{
+    if (m.stats && v && (threadIdx.x & 63) == 0) atomicAdd(&m.stats[slot], (unsigned long long)v);
+}
+
 // Upper bound of one query's squared distance: a greedy descent (nearer child box at every level,
 // per lane) to one leaf, then that leaf's exact distance (the leaf test of the walks).  It is a real
 // candidate, so seeding a walk with it leaves the result unchanged while every lane prunes from the
@@ -560,6 +566,7 @@ __device__ __forceinline__ float coopDistanceWalk(const MeshDev& m, float px, fl
     const int lane = threadIdx.x & 63;
     stopped = false;
     int sp = 0, steps = 0, qn = 0;
+    uint32_t nQuad = 0, nLeaf = 0, nTest = 0;     // measurement counts
     int refs = 0;      // lane e: node ref of stack entry e
     bool have = true;
     stk->best[lane] = __float_as_uint(best);
@@ -582,8 +589,10 @@ __device__ __forceinline__ float coopDistanceWalk(const MeshDev& m, float px, fl
             // lanes whose triangle may still lie closer than their best (the leaf test decides)
             const uint64_t may =
                 FILTER ? ~(uint64_t)__builtin_amdgcn_fcmpf(leafLowerBound(m, ~cur, px, py, pz), bestHi, 2) : ~0ull;
+            nLeaf += FILTER ? 1u : 0u;
             act &= may;
             if (qn + __popcll(act) > 64) {
+                nTest += qn;
                 best = flushLeaves<D>(m, px, py, pz, stk, qn);
                 bestHi = boundAbove(best);
                 qn = 0;
@@ -599,6 +608,7 @@ __device__ __forceinline__ float coopDistanceWalk(const MeshDev& m, float px, fl
             continue;
         }
         const Quad Q = loadQuad(m, cur);
+        ++nQuad;
         float d[4];
         uint64_t b[4];
 #pragma unroll
@@ -640,6 +650,12 @@ __device__ __forceinline__ float coopDistanceWalk(const MeshDev& m, float px, fl
         have = true;
     }
     if (qn) best = flushLeaves<D>(m, px, py, pz, stk, qn);
+    if (m.stats) {
+        statAdd(m, kStatWalks, 1);
+        statAdd(m, kStatQuadSteps, nQuad);
+        statAdd(m, kStatPrismReads, nLeaf);
+        statAdd(m, kStatLeafTests, nTest + (uint32_t)qn);
+    }
     return best;
 }
 
@@ -659,6 +675,8 @@ __device__ __forceinline__ float coopDistance(const MeshDev& m, float px, float 
         const float g = greedyLeafDistSq(m, px, py, pz);
         if (g < best) best = g;
     }
+    statAdd(m, kStatSeeds, 1);
+    statAdd(m, kStatSeedLevels, (uint32_t)m.depth);
     const float rootd = boxDistSq(m.rootMin[0], m.rootMin[1], m.rootMin[2], m.rootMax[0], m.rootMax[1],
                                   m.rootMax[2], px, py, pz);
     bool st = false;
@@ -681,9 +699,11 @@ __device__ __forceinline__ float coopWinding(const MeshDev& m, float qx, float q
     uint64_t curMask = waveBallot(valid);
     int cur = m.rootRef;
     int sp = 0;
+    uint32_t nVisit = 0, nDesc = 0, nLeaf = 0;     // measurement counts
     while (curMask) {
         const bool mine = (curMask >> lane) & 1;
         cur = uniform(cur);
+        ++nVisit;
         float4 d0, d1;
         if (cur >= 0) { d0 = cload(m.dint, 2 * cur); d1 = cload(m.dint, 2 * cur + 1); }
         else { d0 = cload(m.dleaf, 2 * (~cur)); d1 = cload(m.dleaf, 2 * (~cur) + 1); }
@@ -700,10 +720,12 @@ __device__ __forceinline__ float coopWinding(const MeshDev& m, float qx, float q
                 const float4 n3 = cload(m.dnode, 4 * cur + 3);
                 stk[sp] = make_uint4(__float_as_uint(n3.y), (uint32_t)desc, (uint32_t)(desc >> 32), 0u);
                 ++sp;
+                ++nDesc;
                 cur = __float_as_int(n3.x);
                 curMask = desc;
                 continue;
             }
+            ++nLeaf;
             const Leaf L = loadLeaf(m.tri, ~cur);
             if (mine && !approx && L.face != skipFace) res += solidAngle(qx, qy, qz, L);
         }
@@ -712,6 +734,12 @@ __device__ __forceinline__ float coopWinding(const MeshDev& m, float qx, float q
         const uint4 e = stk[sp];
         cur = (int)e.x;
         curMask = (uint64_t)e.y | ((uint64_t)e.z << 32);
+    }
+    if (m.stats) {
+        statAdd(m, kStatWalks, 1);
+        statAdd(m, kStatFwnVisits, nVisit);
+        statAdd(m, kStatFwnDescents, nDesc);
+        statAdd(m, kStatFwnLeaves, nLeaf);
     }
     return kInv4Pi * res;
 }
@@ -807,6 +835,7 @@ __device__ __forceinline__ void coopProjectWalk(const MeshDev& m, float px, floa
     stopped = false;
     const int lane = threadIdx.x & 63;
     int sp = 0, steps = 0, qn = 0;
+    uint32_t nStep = 0, nTest = 0;     // measurement counts
     stk->best[lane] = ((unsigned long long)__float_as_uint(res.distSq) << 32) | bestKey;
     stk->win[lane] = -1;
     float bestT = pruneLimit(res.distSq);
@@ -846,6 +875,7 @@ __device__ __forceinline__ void coopProjectWalk(const MeshDev& m, float px, floa
         }
         if (cur < 0) {
             if (qn + __popcll(act) > 64) {
+                nTest += qn;
                 flushProjLeaves<D>(m, px, py, pz, loE, stk, qn, res.distSq, bestKey);
                 bestT = pruneLimit(res.distSq);
                 qn = 0;
@@ -862,6 +892,7 @@ __device__ __forceinline__ void coopProjectWalk(const MeshDev& m, float px, floa
             continue;
         }
         const Quad Q = loadQuad(m, cur);
+        ++nStep;
         // the two children's boxes (the node's binary record) decide this lane's first choice, the
         // grandchildren's its second
         const float4 n0 = cload(m.dnode, 4 * (size_t)cur), n1 = cload(m.dnode, 4 * (size_t)cur + 1);
@@ -939,6 +970,11 @@ __device__ __forceinline__ void coopProjectWalk(const MeshDev& m, float px, floa
         have = true;
     }
     if (qn) flushProjLeaves<D>(m, px, py, pz, loE, stk, qn, res.distSq, bestKey);
+    if (m.stats) {
+        statAdd(m, kStatWalks, 1);
+        statAdd(m, kStatKeyedSteps, nStep);
+        statAdd(m, kStatKeyedTests, nTest + (uint32_t)qn);
+    }
     // the record of the winning leaf (its distance is the one just merged, recomputed identically)
     const int wl = stk->win[lane];
     if (wl >= 0) {
@@ -982,6 +1018,8 @@ __device__ __forceinline__ void coopProject(const MeshDev& m, float px, float py
     // search radius just above the greedy leaf's distance: that leaf (and every leaf at the minimum)
     // is still accepted, so the record is unchanged; an empty search keeps the caller's limit
     const float loE = effectiveLo(lo, up);
+    statAdd(m, kStatSeeds, 1);
+    statAdd(m, kStatSeedLevels, (uint32_t)m.depth);
     if (valid && up > 0) {
         const float g = loOrder(greedyLeafDistSq(m, px, py, pz), loE);
         if (g < up) {
