@@ -85,7 +85,11 @@ size_t mi_last_error(char* buf, size_t len);
  * entry points at this many bytes (used by tests to force multi-chunk execution) */
 int mi_set_memory_limit(size_t bytes);
 
-/* Traversal form of the kernels, process-wide (no counterpart in the reference; its GPU path
+/* Process-wide settings.  They are atomic, so setting them from any thread is defined, but every
+ * thread shares them: a thread that needs its own value sets it on its meshes with
+ * mi_mesh_set_mode (below), which takes precedence over the process-wide value.
+ *
+ * Traversal form of the kernels, process-wide default (no counterpart in the reference; its GPU path
  * has one form, MRCudaFastWindingNumber.cu:31-133).  Results do not depend on it.
  *   MI_TRAVERSAL_PER_LANE  one depth-first walk per query, the reference's form
  *   MI_TRAVERSAL_WAVE_COOP one walk per 64-query wave with uniform node loads (default) */
@@ -94,7 +98,7 @@ int mi_set_memory_limit(size_t bytes);
 int mi_set_traversal(int mode);
 int mi_get_traversal(void);
 
-/* Leaf filter of the wave-cooperative signed-distance walks, process-wide (no counterpart in the
+/* Leaf filter of the wave-cooperative signed-distance walks, process-wide default (no counterpart in the
  * reference).  A queued leaf test is dropped for the lanes where a lower bound of the triangle's
  * distance (its plane slab and edge half-spaces, with rounding margins) already exceeds the lane's
  * best, so results do not depend on it.  MI_LEAF_FILTER_AUTO (default) uses it for grids of more
@@ -117,7 +121,7 @@ int mi_get_leaf_filter(void);
 int mi_set_tree_builder(int builder);
 int mi_get_tree_builder(void);
 
-/* Step budgets of the split form of the wave-cooperative walks, process-wide (no counterpart in the
+/* Step budgets of the split form of the wave-cooperative walks, process-wide defaults (no counterpart in the
  * reference).  A 64-query wave still walking after `steps` node visits hands its queries, with their
  * best candidates so far, to a second launch that walks the tree's 64 top subtrees for them in
  * parallel; results are identical for every budget.  0 = never split; MI_SPLIT_AUTO (the default)
@@ -175,6 +179,24 @@ int  mi_mesh_set_stats(mi_mesh* m, uint64_t* d_counters);
 int  mi_mesh_get_tree(const mi_mesh* m, mi_tree_node* nodes, mi_dipole* dipoles);
 /* depth of the tree (longest root-to-leaf path, in edges) */
 int  mi_mesh_depth(const mi_mesh* m);
+/* Settings of one mesh (no counterpart in the reference, whose GPU classes carry no mode): each
+ * overrides, for the calls on this mesh only, the process-wide setting of the same name above, so
+ * that threads working on different meshes do not share settings.  Every setting starts at
+ * MI_MODE_DEFAULT, which follows the process-wide setting at each call.  Values:
+ *   MI_MODE_TRAVERSAL          MI_TRAVERSAL_*
+ *   MI_MODE_LEAF_FILTER        MI_LEAF_FILTER_*
+ *   MI_MODE_SPLIT_SDF          steps or MI_SPLIT_AUTO (as mi_set_split_budget(MI_SPLIT_SDF, ...))
+ *   MI_MODE_SPLIT_PROJECTIONS  steps or MI_SPLIT_AUTO
+ * A setting changed while a call on the mesh runs applies from the next call on.  mi_mesh_get_mode
+ * returns the mesh's own value (MI_MODE_DEFAULT when it follows the process-wide setting). */
+#define MI_MODE_TRAVERSAL 0
+#define MI_MODE_LEAF_FILTER 1
+#define MI_MODE_SPLIT_SDF 2
+#define MI_MODE_SPLIT_PROJECTIONS 3
+#define MI_MODE_COUNT 4
+#define MI_MODE_DEFAULT (-2)
+int     mi_mesh_set_mode(mi_mesh* m, int32_t mode, int32_t value);
+int32_t mi_mesh_get_mode(const mi_mesh* m, int32_t mode);
 /* the same host tree + dipole build without any device (AABBTree ctor + calcDipoles):
  * fills nodes[2*k-1] and dipoles[2*k-1], k = the valid faces of tris (either may be NULL), and *depth */
 int  mi_build_tree_host(const float* verts, int32_t nv, const int32_t* tris, int32_t nf,

@@ -111,6 +111,8 @@ def lib():
         "mi_mesh_get_tree": (C.c_int, [P, P, P]),
         "mi_mesh_depth": (C.c_int, [P]),
         "mi_mesh_set_stats": (C.c_int, [P, P]),
+        "mi_mesh_set_mode": (C.c_int, [P, I32, I32]),
+        "mi_mesh_get_mode": (I32, [P, I32]),
         "mi_build_tree_host": (C.c_int, [P, I32, P, I32, P, P, P]),
         "mi_fwn_calc_from_vector": (C.c_int, [P, P, I64, F, I32, P, _PROGRESS, P]),
         "mi_fwn_calc_self_intersections": (C.c_int, [P, F, P, _PROGRESS, P]),
@@ -253,6 +255,20 @@ def get_tree_builder() -> str:
     return {b: a for a, b in TREE_BUILDERS.items()}[v]
 
 
+MESH_MODES = {"traversal": 0, "leaf_filter": 1, "split_sdf": 2, "split_projections": 3}
+MODE_DEFAULT = -2
+
+
+def _mode_value(mode: str, value) -> int:
+    if value is None:
+        return MODE_DEFAULT
+    if mode == "traversal":
+        return TRAVERSALS[value]
+    if mode == "leaf_filter":
+        return LEAF_FILTERS[value]
+    return int(value)
+
+
 def set_memory_limit(nbytes: int):
     """Caps the device-memory budget of the chunked entry points (0 = default policy)."""
     _check(lib().mi_set_memory_limit(int(nbytes)))
@@ -324,6 +340,20 @@ class Mesh:
 
     def depth(self) -> int:
         return int(lib().mi_mesh_depth(self._h))
+
+    def set_mode(self, mode: str, value):
+        """This mesh's own setting, overriding the process-wide one for calls on this mesh only:
+        mode 'traversal' (value as set_traversal), 'leaf_filter' (as set_leaf_filter), 'split_sdf' or
+        'split_projections' (steps, -1 automatic); value None follows the process-wide setting."""
+        _check(lib().mi_mesh_set_mode(self._h, MESH_MODES[mode], _mode_value(mode, value)))
+
+    def get_mode(self, mode: str):
+        """This mesh's own setting (None when it follows the process-wide one)."""
+        v = int(lib().mi_mesh_get_mode(self._h, MESH_MODES[mode]))
+        if v == MODE_DEFAULT:
+            return None
+        names = {"traversal": TRAVERSALS, "leaf_filter": LEAF_FILTERS}.get(mode)
+        return {b: a for a, b in names.items()}[v] if names else v
 
     def set_stats(self, d_counters: int):
         """Measurement hook: device pointer to MI_STAT_LAUNCHES * MI_STAT_STRIDE zeroed uint64 counters

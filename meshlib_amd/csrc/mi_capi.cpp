@@ -189,10 +189,28 @@ struct mi_mesh {
     hipStream_t stream = nullptr;
     std::mutex mu;
     DevBuf ws;   // sort workspace of the device-resident entry points (grow-only)
+    // the mesh's own settings (MI_MODE_*), MI_MODE_DEFAULT = follow the process-wide setting
+    std::atomic<int32_t> modes[MI_MODE_COUNT] = { MI_MODE_DEFAULT, MI_MODE_DEFAULT, MI_MODE_DEFAULT,
+                                                  MI_MODE_DEFAULT };
     ~mi_mesh() { if (stream) (void)hipStreamDestroy(stream); }
 };
 
 namespace {
+
+int32_t meshMode(const mi_mesh* m, int32_t mode, int32_t processDefault)
+{
+    const int32_t v = m->modes[mode].load();
+    return v == MI_MODE_DEFAULT ? processDefault : v;
+}
+
+// the mesh's device view with this call's settings resolved
+mi::MeshDev callView(const mi_mesh* m)
+{
+    mi::MeshDev d = m->dev;
+    d.traversal = meshMode(m, MI_MODE_TRAVERSAL, static_cast<int32_t>(mi::traversal()));
+    d.leafFilter = meshMode(m, MI_MODE_LEAF_FILTER, mi::leafFilter());
+    return d;
+}
 
 int ensureWorkspace(mi_mesh* m, size_t bytes)
 {
@@ -209,15 +227,24 @@ int sdfLaneBytes(const mi_dist_options* o) { return o && o->minDistSq > 0 ? 8 : 
 // (lane bytes + the cut-order byte + the unit list)
 size_t sdfBytesPerVoxel(const mi_dist_options* o) { return o ? sizeof(float) + sdfLaneBytes(o) + 2 : sizeof(float); }
 
+// waves of a signed-distance launch over `layers` z-layers and the split budget it runs with
+int64_t sdfSplitPlan(const mi_mesh* m, const int32_t dims[3], int32_t layers, int32_t* budget)
+{
+    *budget = 0;
+    if (layers <= 0) return 0;
+    const int32_t d[3] = { dims[0], dims[1], layers };
+    const int64_t waves = mi::sdfGridWaves(d, 0, layers);
+    const int setting = meshMode(m, MI_MODE_SPLIT_SDF, g_splitBudget.load());
+    *budget = setting >= 0 ? setting : autoSdfBudget(m->depth, waves);
+    return waves;
+}
+
 // scratch of the split form for grids of up to `layers` z-layers, in `buf` (grown if needed)
 int splitScratch(mi_mesh* m, DevBuf& buf, const int32_t dims[3], int32_t layers, int laneBytes, mi::SplitScratch& sc)
 {
     sc = mi::SplitScratch{};
-    if (layers <= 0) return MI_OK;
-    const int32_t d[3] = { dims[0], dims[1], layers };
-    const int64_t waves = mi::sdfGridWaves(d, 0, layers);
-    const int setting = g_splitBudget.load();
-    const int budget = setting >= 0 ? setting : autoSdfBudget(m->depth, waves);
+    int32_t budget = 0;
+    const int64_t waves = sdfSplitPlan(m, dims, layers, &budget);
     if (budget <= 0) return MI_OK;
     const size_t bytes = mi::splitScratchBytes(waves, laneBytes);
     if (buf.bytes < bytes) MI_HIP(buf.alloc(bytes));
@@ -229,7 +256,7 @@ int splitScratch(mi_mesh* m, DevBuf& buf, const int32_t dims[3], int32_t layers,
 // projSplitBytes); no split when the budget is 0
 int projBudget(const mi_mesh* m)
 {
-    const int setting = g_splitBudgetProj.load();
+    const int setting = meshMode(m, MI_MODE_SPLIT_PROJECTIONS, g_splitBudgetProj.load());
     return setting >= 0 ? setting : autoProjBudget(m->depth);
 }
 
@@ -558,6 +585,28 @@ int mi_mesh_set_stats(mi_mesh* m, uint64_t* d_counters)
 
 int mi_mesh_depth(const mi_mesh* m) { return m ? m->depth : 0; }
 
+int mi_mesh_set_mode(mi_mesh* m, int32_t mode, int32_t value)
+{
+    if (!m) return fail(MI_E_INVALID, "mesh is null");
+    if (mode < 0 || mode >= MI_MODE_COUNT) return fail(MI_E_INVALID, "unknown mesh setting");
+    bool ok = value == MI_MODE_DEFAULT;
+    if (mode == MI_MODE_TRAVERSAL)
+        ok = ok || value == MI_TRAVERSAL_PER_LANE || value == MI_TRAVERSAL_WAVE_COOP;
+    else if (mode == MI_MODE_LEAF_FILTER)
+        ok = ok || value == MI_LEAF_FILTER_OFF || value == MI_LEAF_FILTER_ON || value == MI_LEAF_FILTER_AUTO;
+    else
+        ok = ok || value >= MI_SPLIT_AUTO;
+    if (!ok) return fail(MI_E_INVALID, "invalid value of a mesh setting");
+    m->modes[mode].store(value);
+    return MI_OK;
+}
+
+int32_t mi_mesh_get_mode(const mi_mesh* m, int32_t mode)
+{
+    if (!m || mode < 0 || mode >= MI_MODE_COUNT) return MI_MODE_DEFAULT;
+    return m->modes[mode].load();
+}
+
 int mi_build_tree_host(const float* verts, int32_t nv, const int32_t* tris, int32_t nf, mi_tree_node* nodes,
                        mi_dipole* dipoles, int32_t* depth)
 {
@@ -615,7 +664,7 @@ int mi_sdf_grid_device(mi_mesh* m, const int32_t dims[3], const float xf[12], co
                     opts->windingNumberBeta };
     mi::SplitScratch sc;
     if (int rc = splitScratch(m, m->ws, dims, zEnd - zBegin, sdfLaneBytes(opts), sc)) return rc;
-    MI_HIP(mi::launchSdfGrid(m->dev, dims, toXf(xf), o, zBegin, zEnd, d_res, m->stackD, pick(m, stream), nullptr,
+    MI_HIP(mi::launchSdfGrid(callView(m), dims, toXf(xf), o, zBegin, zEnd, d_res, m->stackD, pick(m, stream), nullptr,
                              nullptr, nullptr, &sc));
     return MI_OK;
 }
@@ -628,7 +677,7 @@ int mi_fwn_grid_device(mi_mesh* m, const int32_t dims[3], const float xf[12], fl
     if (zBegin < 0 || zEnd > dims[2] || zBegin > zEnd) return fail(MI_E_INVALID, "bad z-layer range");
     if (zEnd > zBegin && (int64_t)dims[0] * dims[1] > 0 && !d_res) return fail(MI_E_INVALID, "output is null");
     MI_HIP(hipSetDevice(m->device));
-    MI_HIP(mi::launchFwnGrid(m->dev, dims, toXf(xf), beta, zBegin, zEnd, d_res, m->stackD, pick(m, stream)));
+    MI_HIP(mi::launchFwnGrid(callView(m), dims, toXf(xf), beta, zBegin, zEnd, d_res, m->stackD, pick(m, stream)));
     return MI_OK;
 }
 
@@ -644,7 +693,7 @@ int mi_fwn_vector_device(mi_mesh* m, const float* d_pts, int64_t n, float beta, 
     if (int rc = ensureWorkspace(m, mi::sortWorkspaceBytes(n))) return rc;
     const uint32_t* perm = nullptr;
     MI_HIP(mi::sortPointsMorton(d_pts, n, m->sortLo, m->sortHi, m->ws.p, m->ws.bytes, &perm, s));
-    MI_HIP(mi::launchFwnPoints(m->dev, d_pts, perm, n, beta, skipFace, d_res, m->stackD, s));
+    MI_HIP(mi::launchFwnPoints(callView(m), d_pts, perm, n, beta, skipFace, d_res, m->stackD, s));
     return MI_OK;
 }
 
@@ -666,7 +715,7 @@ int mi_proj_find_projections_device(mi_mesh* m, const float* d_pts, int64_t n, c
     mi::Xf px, tx;
     if (ptXf) px = toXf(ptXf);
     if (treeXf) tx = toXf(treeXf);
-    MI_HIP(mi::launchProjections(m->dev, d_pts, perm, n, ptXf ? &px : nullptr, treeXf ? &tx : nullptr, up, lo,
+    MI_HIP(mi::launchProjections(callView(m), d_pts, perm, n, ptXf ? &px : nullptr, treeXf ? &tx : nullptr, up, lo,
                                  reinterpret_cast<mi::ProjRec*>(d_res), m->stackD, s, nullptr, nullptr, nullptr, &sc));
     return MI_OK;
 }
@@ -690,7 +739,7 @@ int mi_proj_find_projections_bounded_device(mi_mesh* m, const float* d_pts, int6
     mi::Xf px, tx;
     if (ptXf) px = toXf(ptXf);
     if (treeXf) tx = toXf(treeXf);
-    MI_HIP(mi::launchProjections(m->dev, d_pts, perm, n, ptXf ? &px : nullptr, treeXf ? &tx : nullptr, up, lo,
+    MI_HIP(mi::launchProjections(callView(m), d_pts, perm, n, ptXf ? &px : nullptr, treeXf ? &tx : nullptr, up, lo,
                                  reinterpret_cast<mi::ProjRec*>(d_res), m->stackD, s, d_upSq, nullptr, nullptr, &sc));
     return MI_OK;
 }
@@ -705,21 +754,25 @@ int mi_proj_find_projections_capped_device(mi_mesh* m, const float* d_pts, int64
     std::lock_guard<std::mutex> lk(m->mu);
     MI_HIP(hipSetDevice(m->device));
     hipStream_t s = pick(m, stream);
+    // workspace: sort, rerun list, its counter, split scratch
     const size_t sortBytes = align256(mi::sortWorkspaceBytes(n));
-    if (int rc = ensureWorkspace(m, sortBytes + align256((size_t)n * 4) + 256)) return rc;
+    const size_t listBytes = align256((size_t)n * 4) + 256;
+    if (int rc = ensureWorkspace(m, sortBytes + listBytes + projSplitBytes(m, n))) return rc;
     char* base = m->ws.as<char>();
     uint32_t* list = reinterpret_cast<uint32_t*>(base + sortBytes);
     uint32_t* count = reinterpret_cast<uint32_t*>(base + sortBytes + align256((size_t)n * 4));
+    mi::SplitScratch sc;
+    projSplit(m, base + sortBytes + listBytes, n, sc);
     const uint32_t* perm = nullptr;
     MI_HIP(mi::sortPointsMorton(d_pts, n, m->sortLo, m->sortHi, m->ws.p, sortBytes, &perm, s));
     MI_HIP(mi::fillU32(count, 1, 0u, s));
-    MI_HIP(mi::launchProjections(m->dev, d_pts, perm, n, nullptr, nullptr, up, 0.f,
-                                 reinterpret_cast<mi::ProjRec*>(d_res), m->stackD, s, d_capSq, list, count));
+    MI_HIP(mi::launchProjections(callView(m), d_pts, perm, n, nullptr, nullptr, up, 0.f,
+                                 reinterpret_cast<mi::ProjRec*>(d_res), m->stackD, s, d_capSq, list, count, &sc));
     uint32_t h = 0;
     MI_HIP(hipMemcpyAsync(&h, count, 4, hipMemcpyDeviceToHost, s));
     MI_HIP(hipStreamSynchronize(s));
     if (h)
-        MI_HIP(mi::launchProjections(m->dev, d_pts, list, (int64_t)h, nullptr, nullptr, up, 0.f,
+        MI_HIP(mi::launchProjections(callView(m), d_pts, list, (int64_t)h, nullptr, nullptr, up, 0.f,
                                      reinterpret_cast<mi::ProjRec*>(d_res), m->stackD, s));
     if (reruns) *reruns = h;
     return MI_OK;
@@ -741,17 +794,25 @@ int mi_sdf_grid_capped_device(mi_mesh* m, const int32_t dims[3], const float xf[
     std::lock_guard<std::mutex> lk(m->mu);
     MI_HIP(hipSetDevice(m->device));
     hipStream_t s = pick(m, stream);
-    if (int rc = ensureWorkspace(m, align256(nvox * 4) + 256)) return rc;
+    // workspace: rerun list, its counter, split scratch
+    const size_t listBytes = align256(nvox * 4) + 256;
+    int32_t budget = 0;
+    const int64_t waves = sdfSplitPlan(m, dims, zEnd - zBegin, &budget);
+    const size_t splitBytes = budget > 0 ? mi::splitScratchBytes(waves, sdfLaneBytes(opts)) : 0;
+    if (int rc = ensureWorkspace(m, listBytes + splitBytes)) return rc;
     uint32_t* list = m->ws.as<uint32_t>();
     uint32_t* count = reinterpret_cast<uint32_t*>(m->ws.as<char>() + align256(nvox * 4));
+    mi::SplitScratch sc;
+    if (budget > 0) mi::splitScratchBind(m->ws.as<char>() + listBytes, waves, sdfLaneBytes(opts), budget, sc);
     mi::DistOpts o{ opts->minDistSq, opts->maxDistSq, opts->nullOutsideMinMax, opts->windingNumberThreshold,
                     opts->windingNumberBeta };
     MI_HIP(mi::fillU32(count, 1, 0u, s));
-    MI_HIP(mi::launchSdfGrid(m->dev, dims, toXf(xf), o, zBegin, zEnd, d_res, m->stackD, s, d_capSq, list, count));
+    MI_HIP(mi::launchSdfGrid(callView(m), dims, toXf(xf), o, zBegin, zEnd, d_res, m->stackD, s, d_capSq, list, count,
+                             &sc));
     uint32_t h = 0;
     MI_HIP(hipMemcpyAsync(&h, count, 4, hipMemcpyDeviceToHost, s));
     MI_HIP(hipStreamSynchronize(s));
-    if (h) MI_HIP(mi::launchSdfVoxels(m->dev, dims, toXf(xf), o, zBegin, list, (int64_t)h, d_res, m->stackD, s));
+    if (h) MI_HIP(mi::launchSdfVoxels(callView(m), dims, toXf(xf), o, zBegin, list, (int64_t)h, d_res, m->stackD, s));
     if (reruns) *reruns = h;
     return MI_OK;
 }
@@ -787,10 +848,10 @@ static int gridHost(mi_mesh* m, const int32_t dims[3], const float xf[12], const
         if (opts) {
             mi::DistOpts o{ opts->minDistSq, opts->maxDistSq, opts->nullOutsideMinMax,
                             opts->windingNumberThreshold, opts->windingNumberBeta };
-            MI_HIP(mi::launchSdfGrid(m->dev, dims, x, o, z0, z1, out.as<float>(), m->stackD, m->stream, nullptr,
+            MI_HIP(mi::launchSdfGrid(callView(m), dims, x, o, z0, z1, out.as<float>(), m->stackD, m->stream, nullptr,
                                      nullptr, nullptr, &sc));
         } else {
-            MI_HIP(mi::launchFwnGrid(m->dev, dims, x, beta, z0, z1, out.as<float>(), m->stackD, m->stream));
+            MI_HIP(mi::launchFwnGrid(callView(m), dims, x, beta, z0, z1, out.as<float>(), m->stackD, m->stream));
         }
         MI_HIP(hipMemcpyAsync(res + (size_t)z0 * layer, out.p, (size_t)(z1 - z0) * layer * sizeof(float),
                               hipMemcpyDeviceToHost, m->stream));
@@ -883,10 +944,10 @@ static int gridByParts(mi_mesh* m, mi_grid_part_fn fn, void* fnUser, const int32
         if (opts) {
             mi::DistOpts o{ opts->minDistSq, opts->maxDistSq, opts->nullOutsideMinMax,
                             opts->windingNumberThreshold, opts->windingNumberBeta };
-            MI_HIP(mi::launchSdfGrid(m->dev, dims, x, o, z0, z1, dbuf[b].as<float>(), m->stackD, st[b], nullptr,
+            MI_HIP(mi::launchSdfGrid(callView(m), dims, x, o, z0, z1, dbuf[b].as<float>(), m->stackD, st[b], nullptr,
                                      nullptr, nullptr, &sc[b]));
         } else {
-            MI_HIP(mi::launchFwnGrid(m->dev, dims, x, beta, z0, z1, dbuf[b].as<float>(), m->stackD, st[b]));
+            MI_HIP(mi::launchFwnGrid(callView(m), dims, x, beta, z0, z1, dbuf[b].as<float>(), m->stackD, st[b]));
         }
         MI_HIP(hipMemcpyAsync(hbuf[b], dbuf[b].p, (size_t)(z1 - z0) * layer * sizeof(float),
                               hipMemcpyDeviceToHost, st[b]));
@@ -985,7 +1046,7 @@ int mi_fwn_calc_from_vector(mi_mesh* m, const float* pts, int64_t n, float beta,
     if (!m || n < 0 || (n > 0 && (!pts || !res))) return fail(MI_E_INVALID, "invalid arguments");
     return pointsHost(m, pts, n, sizeof(float), res,
                       [&](const float* dp, const uint32_t* perm, int64_t sz, void* dr, const mi::SplitScratch*) {
-                          return mi::launchFwnPoints(m->dev, dp, perm, sz, beta, skipFace, static_cast<float*>(dr),
+                          return mi::launchFwnPoints(callView(m), dp, perm, sz, beta, skipFace, static_cast<float*>(dr),
                                                      m->stackD, m->stream);
                       },
                       cb, user);
@@ -1004,7 +1065,7 @@ int mi_fwn_calc_self_intersections(mi_mesh* m, float beta, uint8_t* res, mi_prog
     DevBuf out;
     MI_HIP(out.alloc((size_t)words * 4));
     MI_HIP(mi::fillU32(out.as<uint32_t>(), words, 0u, m->stream));
-    MI_HIP(mi::launchSelfIntersections(m->dev, beta, out.as<uint8_t>(), m->stackD, m->stream));
+    MI_HIP(mi::launchSelfIntersections(callView(m), beta, out.as<uint8_t>(), m->stackD, m->stream));
     MI_HIP(hipMemcpyAsync(res, out.p, (size_t)m->faceSize, hipMemcpyDeviceToHost, m->stream));
     MI_HIP(hipStreamSynchronize(m->stream));
     if (!report(cb, user, 1.0f)) return fail(MI_E_CANCELED, "Operation was canceled");
@@ -1032,7 +1093,7 @@ int mi_proj_find_projections(mi_mesh* m, const float* pts, int64_t n, const floa
     (void)kIdentityXf;
     return pointsHost(m, pts, n, sizeof(mi_projection), res,
                       [&](const float* dp, const uint32_t* perm, int64_t sz, void* dr, const mi::SplitScratch* sc) {
-                          return mi::launchProjections(m->dev, dp, perm, sz, ptXf ? &px : nullptr,
+                          return mi::launchProjections(callView(m), dp, perm, sz, ptXf ? &px : nullptr,
                                                        treeXf ? &tx : nullptr, up, lo,
                                                        static_cast<mi::ProjRec*>(dr), m->stackD, m->stream,
                                                        nullptr, nullptr, nullptr, sc);

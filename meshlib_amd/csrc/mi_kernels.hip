@@ -192,11 +192,16 @@ __global__ __launch_bounds__(kBlock) void kSdfSplit(MeshDev m, int dx, int dy, i
     }
 }
 
-// Third launch: the handed-over bricks' results (sign from the winding number, NaN band, store).
+// Third launch: the handed-over bricks' results (sign from the winding number, NaN band, store).  With
+// caps (bound) and rerun lists, a voxel that found nothing below its cap is appended instead of stored,
+// as in kSdfGridCoop.
 template <int D>
 __global__ __launch_bounds__(kBlock) void kSdfSplitFinish(MeshDev m, int dx, int dy, int zBegin, int zEnd,
                                                             int nbx, int nby, int nbz, Xf xf, DistOpts o,
-                                                            SplitScratch sc, float* __restrict__ out)
+                                                            SplitScratch sc, const float* __restrict__ bound,
+                                                            uint32_t* __restrict__ rerun,
+                                                            uint32_t* __restrict__ rerunCount,
+                                                            float* __restrict__ out)
 {
     __shared__ float4 stkMem[kWaves][2 * quadStack(D)];
     const int lane = threadIdx.x & 63;
@@ -214,7 +219,11 @@ __global__ __launch_bounds__(kBlock) void kSdfSplitFinish(MeshDev m, int dx, int
         xfApply(xf, px, py, pz);
         const size_t idx = valid ? ((size_t)(z - zBegin) * dy + y) * dx + x : 0;
         const float resSq = __uint_as_float(sc.best[(size_t)t * 64 + lane]);
-        sdfStore(m, px, py, pz, valid, resSq, o, reinterpret_cast<uint4*>(stkMem[w]), out, idx);
+        float lim = o.maxDistSq;
+        if (bound && valid) { const float b = bound[idx]; lim = b < lim ? b : lim; }
+        const bool again = rerun && valid && !(resSq < lim) && lim < o.maxDistSq;
+        if (rerun) appendWave(again, (uint32_t)idx, rerun, rerunCount);
+        sdfStore(m, px, py, pz, valid && !again, resSq, o, reinterpret_cast<uint4*>(stkMem[w]), out, idx);
     }
 }
 
@@ -673,20 +682,20 @@ hipError_t launchSdfGrid(const MeshDev& m, const int32_t dims[3], const Xf& xf, 
     // the wave-cooperative distance returns the minimum over candidates; with minDistSq > 0 the
     // reference stops at the first candidate <= minDistSq in its own visit order, which the keyed walk
     // reproduces (launchSdfGridKeyed; its split scratch holds 8 bytes per lane)
-    if (o.minDistSq > 0 && !bound && traversal() == Traversal::WaveCoop)
+    if (o.minDistSq > 0 && !bound && m.traversal == (int)Traversal::WaveCoop)
         return launchSdfGridKeyed(m, dims, xf, o, zBegin, zEnd, out, stackD, s, split);
-    const bool coop = (traversal() == Traversal::WaveCoop || bound) && !(o.minDistSq > 0);
+    const bool coop = (m.traversal == (int)Traversal::WaveCoop || bound) && !(o.minDistSq > 0);
     if (bound && !coop) return hipErrorInvalidValue;
     // The leaf filter saves instructions but adds a dependent load to every leaf visit.  It pays unless
     // a deep tree meets a small grid: then few waves walk long chains and nothing hides that load.
     // Measured on / off (bumpy spheres): k=5 64^3 0.89, k=7 128^3 0.87, k=8 96^3 0.96, k=8 256^3 0.87,
     // k=9 192^3 0.99 -- but k=8 64^3 (4 k waves) 1.11 and k=9 128^3 (32 k waves) 1.05; on where the grid
     // has more than 2^(depth - 8) waves.
-    const int filterMode = leafFilter();
+    const int filterMode = m.leafFilter;
     const int64_t waves = sdfGridWaves(dims, zBegin, zEnd);
     const bool filter = filterMode < 0 ? waves > ((int64_t)1 << (m.depth > 8 ? m.depth - 8 : 0)) : filterMode != 0;
     SplitScratch sc;
-    if (coop && split && split->list && split->budget > 0 && !rerun && m.nCut > 0) {
+    if (coop && split && split->list && split->budget > 0 && m.nCut > 0) {
         if ((int64_t)split->cap < sdfGridWaves(dims, zBegin, zEnd)) return hipErrorInvalidValue;
         sc = *split;
         hipError_t e = fillU32(sc.counters, 3, 0u, s);
@@ -704,7 +713,7 @@ hipError_t launchSdfGrid(const MeshDev& m, const int32_t dims[3], const Xf& xf, 
                 hipLaunchKernelGGL(splitKernel, dim3(blocks), dim3(kBlock), 0, s, statLaunch(m, 1), dims[0], dims[1], zBegin, zEnd,
                                    nbx, nby, nbz, xf, sc);
                 hipLaunchKernelGGL(kSdfSplitFinish<D>, dim3(blocks), dim3(kBlock), 0, s, statLaunch(m, 2), dims[0], dims[1],
-                                   zBegin, zEnd, nbx, nby, nbz, xf, o, sc, out);
+                                   zBegin, zEnd, nbx, nby, nbz, xf, o, sc, bound, rerun, rerunCount, out);
             }
         } else {
             hipLaunchKernelGGL(kSdfGrid<D>, dim3(chunkedGrid(nLogical)), dim3(kBlock), 0, s, m, dims[0],
@@ -736,7 +745,7 @@ hipError_t launchFwnGrid(const MeshDev& m, const int32_t dims[3], const Xf& xf, 
     const int64_t bricks = (int64_t)nbx * nby * nbz;
     if (bricks <= 0) return hipSuccess;
     const int nLogical = (int)((bricks + 3) / 4);
-    const bool coop = traversal() == Traversal::WaveCoop;
+    const bool coop = m.traversal == (int)Traversal::WaveCoop;
     return dispatchDepth(stackD, [&](auto Dc) {
         constexpr int D = decltype(Dc)::value;
         if (coop)
@@ -754,7 +763,7 @@ hipError_t launchFwnPoints(const MeshDev& m, const float* pts, const uint32_t* p
 {
     if (n <= 0) return hipSuccess;
     const int nLogical = (int)((n + kBlock - 1) / kBlock);
-    const bool coop = traversal() == Traversal::WaveCoop;
+    const bool coop = m.traversal == (int)Traversal::WaveCoop;
     return dispatchDepth(stackD, [&](auto Dc) {
         constexpr int D = decltype(Dc)::value;
         if (coop)
@@ -771,7 +780,7 @@ hipError_t launchSelfIntersections(const MeshDev& m, float beta, uint8_t* out, i
 {
     if (m.numFaces <= 0) return hipSuccess;
     const int nLogical = (m.numFaces + kBlock - 1) / kBlock;
-    const bool coop = traversal() == Traversal::WaveCoop;
+    const bool coop = m.traversal == (int)Traversal::WaveCoop;
     return dispatchDepth(stackD, [&](auto Dc) {
         constexpr int D = decltype(Dc)::value;
         if (coop)

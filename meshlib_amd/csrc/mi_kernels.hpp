@@ -53,6 +53,10 @@ struct MeshDev {
     int32_t numFaces;
     int32_t depth;         // levels of the tree
     unsigned long long* stats;   // measurement counters (kStatStride per launch) or null
+    // settings of the call (host side, read by the launchers only): resolved per call from the mesh's
+    // own settings and the process-wide defaults
+    int32_t traversal = 1;       // Traversal
+    int32_t leafFilter = -1;     // leaf filter mode of the signed-distance walks: 0 off, 1 on, -1 auto
 };
 
 // the mesh view of a call's launch number `launch` (its own block of counters)
@@ -73,6 +77,7 @@ static_assert(sizeof(ProjRec) == 32, "MeshProjectionResult is 32 B");
 // traversal form: PerLane = one independent depth-first walk per lane (the reference's form);
 // WaveCoop = one walk per wave over its 64 coherent queries (results identical, see kernels)
 enum class Traversal : int { PerLane = 0, WaveCoop = 1 };
+// process-wide defaults of the call settings in MeshDev
 void setTraversal(Traversal t);
 Traversal traversal();
 // leaf filter of the signed-distance grid walks: 0 off, 1 on, -1 by the grid's wave count

@@ -212,9 +212,13 @@ __global__ __launch_bounds__(kBlock) void kKeyedSplit(MeshDev m, Src src, SplitS
 }
 
 // Third launch of the split projections: the handed-over waves' records, from the leaf the winning
-// key names (its closest point and true squared distance recomputed exactly as the leaf test).
+// key names (its closest point and true squared distance recomputed exactly as the leaf test).  With
+// caps (src.bound) and rerun lists, a query that found nothing below its cap is appended instead of
+// written, as in kProjectionsCoop.
 template <int D>
 __global__ __launch_bounds__(kBlock) void kProjSplitFinish(MeshDev m, PointSource src, SplitScratch sc,
+                                                           uint32_t* __restrict__ rerun,
+                                                           uint32_t* __restrict__ rerunCount,
                                                            ProjRec* __restrict__ out)
 {
     const int lane = threadIdx.x & 63;
@@ -228,12 +232,15 @@ __global__ __launch_bounds__(kBlock) void kProjSplitFinish(MeshDev m, PointSourc
         int64_t q;
         float px, py, pz, lim;
         const bool valid = src.get(sc.list[t], lane, q, px, py, pz, lim);
-        if (!valid) continue;
-        const unsigned long long b = best[(size_t)t * 64 + lane];
+        const unsigned long long b = valid ? best[(size_t)t * 64 + lane] : 0ull;
         const float e = __uint_as_float((uint32_t)(b >> 32));
+        // a winner has d < lim, i.e. order value < lim (loE < lim); otherwise the reference's empty result,
+        // or (a cap below up) a rerun without the cap
+        const int leaf = valid && e < lim ? keyLeaf(m, (uint32_t)b, px, py, pz) : -1;
+        const bool again = rerun && valid && leaf < 0 && lim < src.up;
+        if (rerun) appendWave(again, (uint32_t)q, rerun, rerunCount);
+        if (!valid || again) continue;
         float4* o = reinterpret_cast<float4*>(out + q);
-        // a winner has d < lim, i.e. order value < lim (loE < lim); otherwise the reference's empty result
-        const int leaf = e < lim ? keyLeaf(m, (uint32_t)b, px, py, pz) : -1;
         if (leaf < 0) {
             o[0] = make_float4(__int_as_float(-1), 0.f, 0.f, 0.f);
             o[1] = make_float4(__int_as_float(-1), 0.f, 0.f, lim);
@@ -329,10 +336,10 @@ hipError_t launchProjections(const MeshDev& m, const float* pts, const uint32_t*
     // the keyed walk reproduces the reference's record for every loDistLimitSq; a non-rigid tree
     // transform (applied per box) keeps the per-lane form
     const bool coopOk = !treeXf;
-    const bool coop = coopOk && (traversal() == Traversal::WaveCoop || rerun);
+    const bool coop = coopOk && (m.traversal == (int)Traversal::WaveCoop || rerun);
     if (rerun && !coop) return hipErrorInvalidValue;
     SplitScratch sc;
-    if (coop && split && split->list && split->budget > 0 && !rerun && m.nCut > 0) {
+    if (coop && split && split->list && split->budget > 0 && m.nCut > 0) {
         if ((int64_t)split->cap < projectionWaves(n)) return hipErrorInvalidValue;
         sc = *split;
         hipError_t e = fillU32(sc.counters, 3, 0u, s);
@@ -349,7 +356,7 @@ hipError_t launchProjections(const MeshDev& m, const float* pts, const uint32_t*
                 hipLaunchKernelGGL((kKeyedSplit<D, PointSource>), dim3(blocks), dim3(kBlock), 0, s, statLaunch(m, 1),
                                    src, sc);
                 hipLaunchKernelGGL(kProjSplitFinish<D>, dim3(blocks), dim3(kBlock), 0, s, statLaunch(m, 2), src, sc,
-                                   out);
+                                   rerun, rerunCount, out);
             }
         } else if (treeXf)
             hipLaunchKernelGGL((kProjections<true, D>), dim3(paddedGrid(nLogical)), dim3(kBlock), 0, s,

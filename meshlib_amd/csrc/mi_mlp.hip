@@ -319,6 +319,188 @@ __global__ __launch_bounds__(256) void kBackward(Params p, const float* __restri
     }
 }
 
+// ---------------------------------------------------------------- inference
+
+// Inference kernel: persistent workgroups of 8 waves hold all four hidden layers' BF16 weights in LDS
+// (staged once per workgroup, in MFMA fragment order: one 1-KB block per (layer, 16-row tile, 32-wide K
+// chunk), lane l's 16 bytes at 16 l), and each wave carries 64 points (4 groups of 16) through the
+// layers with the activations in registers.  The layers are computed transposed, Z^T = W . H^T: the B
+// operand is the activations (lane l: point l%16 of a group, 8 K elements of a chunk), and an output
+// tile leaves lane l holding 4 features 16 mt + 4(l/16) + j of point l%16.  Two output tiles 2c, 2c+1 are
+// one K chunk of the next layer with its K elements in the order (32c + 4g + 0..3, 32c + 16 + 4g + 0..3)
+// for lane group g = l/16; the staged weights of layers 1..3 are permuted to that order, so no
+// activation leaves its lane.  Layer 0's chunks take the features in natural order (lane group g:
+// features 32c + 8g + 0..7), computed in-lane with the hardware sine / cosine (v_sin_f32, v_cos_f32 take
+// revolutions: sin(pi f x) = v_sin(f x / 2); within 2e-4 of sinf / cosf, far below the features' BF16
+// rounding).  Rounding points as in kForward: features, weights and hidden activations in BF16, FP32
+// accumulation, the output layer on BF16 activations and weights.
+constexpr int kPredWaves = 8;
+constexpr int kPredGroups = 4;                        // 16-point groups per wave
+constexpr int kPredPoints = 16 * kPredGroups;         // points per wave and tile
+constexpr int kFragBlocks = kLayers * 8 * 4;          // (layer, tile, chunk) fragment blocks
+
+// query source of the inference: free points, or the voxel centres of a grid slab (x fastest)
+struct PredSource {
+    const float* pts;      // null: grid
+    int dx, dy, zBegin;
+    float xf[12];
+    __device__ __forceinline__ void point(int64_t i, float p[3]) const
+    {
+        if (pts) {
+            p[0] = pts[3 * i]; p[1] = pts[3 * i + 1]; p[2] = pts[3 * i + 2];
+            return;
+        }
+        const int64_t layer = (int64_t)dx * dy;
+        const float z = (float)(zBegin + (int)(i / layer));
+        const int rem = (int)(i % layer);
+        const float x = (float)(rem % dx), y = (float)(rem / dx);
+        p[0] = (xf[0] * x + xf[1] * y + xf[2] * z) + xf[9];
+        p[1] = (xf[3] * x + xf[4] * y + xf[5] * z) + xf[10];
+        p[2] = (xf[6] * x + xf[7] * y + xf[8] * z) + xf[11];
+    }
+};
+
+__device__ __forceinline__ float siluFast(float z) { return z * __builtin_amdgcn_rcpf(1.f + __expf(-z)); }
+
+// feature f of the normalised point with the hardware sine / cosine; halfFreq[k] = f_k / 2
+__device__ __forceinline__ float featureFast(int f, const float x[3], const float* halfFreq)
+{
+    if (f < 3) return x[f];
+    if (f >= kFeatures) return 0.f;
+    const int j = f - 3, k = j / 6, r = j % 6;
+    const float a = halfFreq[k] * x[r < 3 ? r : r - 3];
+    return r < 3 ? __builtin_amdgcn_sinf(a) : __builtin_amdgcn_cosf(a);
+}
+
+__device__ __forceinline__ uint32_t pack2(uint16_t lo, uint16_t hi) { return (uint32_t)lo | ((uint32_t)hi << 16); }
+
+__global__ __launch_bounds__(64 * kPredWaves) void kPredict(Params p, PredSource src, int64_t n,
+                                                             float* __restrict__ out, float capMargin)
+{
+    __shared__ __attribute__((aligned(16))) uint4 Wf[kFragBlocks * 64];
+    __shared__ float bias[kLayers * kHidden];
+    __shared__ float w5[kHidden];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int lr = lane & 15, lg = lane >> 4;
+    // stage the weights in fragment order (layers 1.. with the permuted K order)
+    for (int e = tid; e < kFragBlocks * 64; e += 64 * kPredWaves) {
+        const int ln = e & 63, blk = e >> 6;
+        const int kc = blk & 3, mt = (blk >> 2) & 7, l = blk >> 5;
+        const int o = 16 * mt + (ln & 15), g = ln >> 4;
+        const uint16_t* row = p.Wb + ((size_t)l * kHidden + o) * kHidden;
+        uint4 v;
+        if (l == 0) {
+            v = *reinterpret_cast<const uint4*>(row + 32 * kc + 8 * g);
+        } else {
+            const uint2 a = *reinterpret_cast<const uint2*>(row + 32 * kc + 4 * g);
+            const uint2 b = *reinterpret_cast<const uint2*>(row + 32 * kc + 16 + 4 * g);
+            v = make_uint4(a.x, a.y, b.x, b.y);
+        }
+        Wf[e] = v;
+    }
+    for (int e = tid; e < kLayers * kHidden; e += 64 * kPredWaves) bias[e] = p.theta[kOffB + e];
+    for (int e = tid; e < kHidden; e += 64 * kPredWaves) w5[e] = bf2f(p.w5b[e]);
+    __syncthreads();
+    float halfFreq[kBands];
+#pragma unroll
+    for (int k = 0; k < kBands; ++k) halfFreq[k] = p.piFreq[k] * (float)(0.5 / M_PI);
+    const float b5 = p.theta[kOffB5];
+    const int64_t tiles = (n + kPredPoints - 1) / kPredPoints;
+    for (int64_t t = (int64_t)blockIdx.x * kPredWaves + wv; t < tiles; t += (int64_t)gridDim.x * kPredWaves) {
+        const int64_t base = t * kPredPoints;
+        // layer 0 inputs: this lane's 32 features of its point in each group
+        bf16x8 cur[kPredGroups][4];
+#pragma unroll
+        for (int g = 0; g < kPredGroups; ++g) {
+            const int64_t i = base + 16 * g + lr;
+            float x[3] = { 0.f, 0.f, 0.f };
+            if (i < n) {
+                float pt[3];
+                src.point(i, pt);
+                normalised(p, pt, x);
+            }
+#pragma unroll
+            for (int kc = 0; kc < 4; ++kc) {
+                uint16_t h[8];
+#pragma unroll
+                for (int e = 0; e < 8; ++e) h[e] = f2bf(featureFast(32 * kc + 8 * lg + e, x, halfFreq));
+                const uint4 v = make_uint4(pack2(h[0], h[1]), pack2(h[2], h[3]), pack2(h[4], h[5]), pack2(h[6], h[7]));
+                cur[g][kc] = __builtin_bit_cast(bf16x8, v);
+            }
+        }
+        for (int l = 0; l < kLayers; ++l) {
+            bf16x8 nxt[kPredGroups][4];
+            const uint4* Wl = Wf + (size_t)l * 32 * 64;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                f32x4 acc[kPredGroups][2];
+#pragma unroll
+                for (int half = 0; half < 2; ++half) {
+                    const int mt = 2 * c + half;
+#pragma unroll
+                    for (int g = 0; g < kPredGroups; ++g) acc[g][half] = f32x4{ 0.f, 0.f, 0.f, 0.f };
+#pragma unroll
+                    for (int kc = 0; kc < 4; ++kc) {
+                        const bf16x8 a = __builtin_bit_cast(bf16x8, Wl[(mt * 4 + kc) * 64 + lane]);
+#pragma unroll
+                        for (int g = 0; g < kPredGroups; ++g) acc[g][half] = mfma(a, cur[g][kc], acc[g][half]);
+                    }
+                }
+                // bias + SiLU, BF16: tile 2c gives K slots 0..3 of chunk c, tile 2c+1 slots 4..7
+                float bb[2][4];
+#pragma unroll
+                for (int half = 0; half < 2; ++half)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) bb[half][j] = bias[l * kHidden + 16 * (2 * c + half) + 4 * lg + j];
+#pragma unroll
+                for (int g = 0; g < kPredGroups; ++g) {
+                    uint16_t h[8];
+#pragma unroll
+                    for (int half = 0; half < 2; ++half)
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) h[4 * half + j] = f2bf(siluFast(acc[g][half][j] + bb[half][j]));
+                    const uint4 v = make_uint4(pack2(h[0], h[1]), pack2(h[2], h[3]), pack2(h[4], h[5]),
+                                               pack2(h[6], h[7]));
+                    nxt[g][c] = __builtin_bit_cast(bf16x8, v);
+                }
+            }
+#pragma unroll
+            for (int g = 0; g < kPredGroups; ++g)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) cur[g][c] = nxt[g][c];
+        }
+        // output layer: this lane's 32 features (chunk c, slot 4 half + j = feature 32c + 16 half + 4lg + j),
+        // then the sum over the four lanes of the point
+#pragma unroll
+        for (int g = 0; g < kPredGroups; ++g) {
+            float s = 0.f;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const uint4 v = __builtin_bit_cast(uint4, cur[g][c]);
+                const uint32_t wds[4] = { v.x, v.y, v.z, v.w };
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const uint16_t hb = (uint16_t)(wds[e >> 1] >> (16 * (e & 1)));
+                    const int f = 32 * c + 16 * (e >> 2) + 4 * lg + (e & 3);
+                    s += bf2f(hb) * w5[f];
+                }
+            }
+            s += __shfl_xor(s, 16);
+            s += __shfl_xor(s, 32);
+            const int64_t i = base + 16 * g + lr;
+            if (lg == 0 && i < n) {
+                const float d = softplusf(s + b5) * p.scale;
+                if (capMargin >= 0.f) {
+                    const float cap = d * (1.f + capMargin);
+                    out[i] = cap * cap;
+                } else {
+                    out[i] = d;
+                }
+            }
+        }
+    }
+}
+
 __global__ void kReduce(const float* __restrict__ partial, int nBlocks, float* __restrict__ grad)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -364,31 +546,52 @@ __global__ void kRefreshAll(Params p)
     refresh(p, j, p.theta[j]);
 }
 
-// voxel-centre points of the slab [zBegin, zEnd) in x-fastest order (the network's inputs)
-__global__ void kGridPoints(int dx, int dy, int zBegin, int64_t count, float m0, float m1, float m2, float m3,
-                            float m4, float m5, float m6, float m7, float m8, float b0, float b1, float b2,
-                            float* __restrict__ out)
-{
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= count) return;
-    const int64_t layer = (int64_t)dx * dy;
-    const float z = (float)(zBegin + (int)(i / layer));
-    const int rem = (int)(i % layer);
-    const float x = (float)(rem % dx), y = (float)(rem / dx);
-    out[3 * i] = (m0 * x + m1 * y + m2 * z) + b0;
-    out[3 * i + 1] = (m3 * x + m4 * y + m5 * z) + b1;
-    out[3 * i + 2] = (m6 * x + m7 * y + m8 * z) + b2;
-}
+} // namespace
 
+namespace {
+int predictBlocks(int64_t n)
+{
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
+            hipSuccess || cus <= 0)
+            cus = 256;
+    }
+    const int64_t tiles = (n + kPredPoints - 1) / kPredPoints;
+    const int64_t want = (tiles + kPredWaves - 1) / kPredWaves;
+    return (int)(want < cus ? want : cus);
+}
 } // namespace
 
 hipError_t forward(const Params& p, const float* pts, const uint32_t* perm, int64_t n, float* out, float capMargin,
                    hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
-    const int blocks = (int)((n + kTileRows - 1) / kTileRows);
-    hipLaunchKernelGGL(kForward<false>, dim3(blocks), dim3(256), 0, s, p, pts, perm, n, out, capMargin, nullptr,
-                       nullptr, nullptr);
+    if (perm) {
+        const int blocks = (int)((n + kTileRows - 1) / kTileRows);
+        hipLaunchKernelGGL(kForward<false>, dim3(blocks), dim3(256), 0, s, p, pts, perm, n, out, capMargin, nullptr,
+                           nullptr, nullptr);
+        return hipGetLastError();
+    }
+    PredSource src{};
+    src.pts = pts;
+    hipLaunchKernelGGL(kPredict, dim3(predictBlocks(n)), dim3(64 * kPredWaves), 0, s, p, src, n, out, capMargin);
+    return hipGetLastError();
+}
+
+hipError_t forwardGrid(const Params& p, const int32_t dims[3], const float xf[12], int32_t zBegin, int32_t zEnd,
+                       float* out, float capMargin, hipStream_t s)
+{
+    const int64_t n = (int64_t)dims[0] * dims[1] * (zEnd - zBegin);
+    if (n <= 0) return hipSuccess;
+    PredSource src{};
+    src.pts = nullptr;
+    src.dx = dims[0];
+    src.dy = dims[1];
+    src.zBegin = zBegin;
+    for (int i = 0; i < 12; ++i) src.xf[i] = xf[i];
+    hipLaunchKernelGGL(kPredict, dim3(predictBlocks(n)), dim3(64 * kPredWaves), 0, s, p, src, n, out, capMargin);
     return hipGetLastError();
 }
 
@@ -421,17 +624,6 @@ hipError_t adamStep(const Params& p, const float* grad, float* mom, float* vel, 
 hipError_t refreshCopies(const Params& p, hipStream_t s)
 {
     hipLaunchKernelGGL(kRefreshAll, dim3((kParams + 255) / 256), dim3(256), 0, s, p);
-    return hipGetLastError();
-}
-
-hipError_t gridPoints(const int32_t dims[3], const float xf[12], int32_t zBegin, int32_t zEnd, float* out,
-                      hipStream_t s)
-{
-    const int64_t count = (int64_t)dims[0] * dims[1] * (zEnd - zBegin);
-    if (count <= 0) return hipSuccess;
-    hipLaunchKernelGGL(kGridPoints, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, dims[0], dims[1], zBegin,
-                       count, xf[0], xf[1], xf[2], xf[3], xf[4], xf[5], xf[6], xf[7], xf[8], xf[9], xf[10], xf[11],
-                       out);
     return hipGetLastError();
 }
 

@@ -47,14 +47,15 @@ struct TrainBuffers {
 // written at the point's index (perm optional)
 hipError_t forward(const Params& p, const float* pts, const uint32_t* perm, int64_t n, float* out, float capMargin,
                    hipStream_t s);
+// the same over the voxel centres of the z-slab [zBegin, zEnd) of a grid (x fastest), without
+// materialising the points
+hipError_t forwardGrid(const Params& p, const int32_t dims[3], const float xf[12], int32_t zBegin, int32_t zEnd,
+                       float* out, float capMargin, hipStream_t s);
 // gradient of 1/(2n) sum (pred/scale - label/scale)^2 over the n points into grad[kParamsPadded]
 hipError_t gradient(const Params& p, const float* pts, const float* labels, int64_t n, const TrainBuffers& tb,
                     float* grad, hipStream_t s);
 hipError_t adamStep(const Params& p, const float* grad, float* mom, float* vel, int step, float lr, hipStream_t s);
 hipError_t refreshCopies(const Params& p, hipStream_t s);
-// voxel-centre points of a z-slab (float3 per voxel, x-fastest)
-hipError_t gridPoints(const int32_t dims[3], const float xf[12], int32_t zBegin, int32_t zEnd, float* out,
-                      hipStream_t s);
 
 } // namespace mlp
 } // namespace mi

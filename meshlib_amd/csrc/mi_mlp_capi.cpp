@@ -255,13 +255,10 @@ int mi_sdf_grid_learned_device(mi_mesh* mesh, mi_mlp* m, float margin, const int
     // the network's lock is held through the capped search, which reads the caps from its workspace
     std::lock_guard<std::mutex> lk(m->mu);
     MLP_HIP(hipSetDevice(m->device));
-    const size_t capBytes = align256(nvox * 4);
-    MLP_HIP(m->work.ensure(capBytes + nvox * 12));
+    MLP_HIP(m->work.ensure(align256(nvox * 4)));
     float* caps = m->work.as<float>();
-    float* pts = reinterpret_cast<float*>(m->work.as<char>() + capBytes);
     hipStream_t s = static_cast<hipStream_t>(stream);
-    MLP_HIP(mi::mlp::gridPoints(dims, xf, zBegin, zEnd, pts, s));
-    MLP_HIP(mi::mlp::forward(m->params, pts, nullptr, (int64_t)nvox, caps, margin, s));
+    MLP_HIP(mi::mlp::forwardGrid(m->params, dims, xf, zBegin, zEnd, caps, margin, s));
     return mi_sdf_grid_capped_device(mesh, dims, xf, opts, m->work.as<float>(), zBegin, zEnd, d_res, reruns, stream);
 }
 
@@ -275,19 +272,17 @@ int mi_batch_run_learned_device(mi_batch* b, const mi_batch_item* items, int32_t
     if (count == 0) return MI_OK;
     hipStream_t s = static_cast<hipStream_t>(stream);
     // caps of every item (squared, from its own network), then the capped two-pass batch
-    size_t need = 0, gridTmp = 0;
+    size_t need = 0;
     for (int32_t i = 0; i < count; ++i) {
         const mi_batch_item& it = items[i];
         if (it.dims[0] < 0 || it.dims[1] < 0 || it.dims[2] < 0 || it.numPts < 0)
             return mi_internal_fail(MI_E_INVALID, "invalid batch item");
         const size_t nvox = (size_t)it.dims[0] * it.dims[1] * it.dims[2];
         need += align256(nvox * 4) + align256((size_t)it.numPts * 4);
-        gridTmp = std::max(gridTmp, nvox * 12);
     }
     Buf caps;
-    MLP_HIP(caps.ensure(align256(gridTmp) + need + 256));
-    float* tmp = caps.as<float>();
-    char* base = caps.as<char>() + align256(gridTmp);
+    MLP_HIP(caps.ensure(need + 256));
+    char* base = caps.as<char>();
     std::vector<const float*> capSdf((size_t)count, nullptr), capProj((size_t)count, nullptr);
     size_t off = 0;
     for (int32_t i = 0; i < count; ++i) {
@@ -302,8 +297,7 @@ int mi_batch_run_learned_device(mi_batch* b, const mi_batch_item* items, int32_t
         std::lock_guard<std::mutex> lk(net->mu);
         MLP_HIP(hipSetDevice(net->device));
         if (nvox) {
-            MLP_HIP(mi::mlp::gridPoints(it.dims, it.xf, 0, it.dims[2], tmp, s));
-            MLP_HIP(mi::mlp::forward(net->params, tmp, nullptr, (int64_t)nvox, cs, margin, s));
+            MLP_HIP(mi::mlp::forwardGrid(net->params, it.dims, it.xf, 0, it.dims[2], cs, margin, s));
             capSdf[i] = cs;
         }
         if (it.numPts) {
