@@ -257,6 +257,19 @@ int mi_proj_find_projections_device(mi_mesh* m, const float* d_pts, int64_t n, c
                                     const float* treeXf, float upDistLimitSq, float loDistLimitSq,
                                     mi_projection* d_res, void* stream);
 
+/* The same call, allocating nothing.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops such as an ICP
+ * iteration, harnesses that forbid hipMalloc).  Same arguments and results as
+ * mi_proj_find_projections_device (bit-identical records), but the sort and split workspace lives
+ * in a fixed 96 MiB device array that the library holds on each GPU it runs on until the process
+ * ends, so it takes at most MI_NOALLOC_MAX_POINTS points per call (MI_E_LIMIT beyond).  Every call
+ * on one GPU, on any mesh, shares that GPU's array: calls that overlap in time on one GPU (on
+ * different streams, or from different threads without ordering) race and give wrong results. */
+#define MI_NOALLOC_MAX_POINTS (1 << 21)
+int mi_proj_find_projections_noalloc_device(mi_mesh* m, const float* d_pts, int64_t n, const float* ptXf,
+                                            const float* treeXf, float upDistLimitSq, float loDistLimitSq,
+                                            mi_projection* d_res, void* stream);
+
 /* findProjection with a per-point upDistLimitSq: point i searches below min(d_upSq[i], upDistLimitSq),
  * the pattern of ICP's limitedProjector (source/MRMesh/MRMeshOrPoints.cpp:121-143 calling
  * findProjection, MRMeshProject.cpp:123-127 with upDistLimitSq = the previous distance). */

@@ -25,7 +25,7 @@ LIB = os.path.join(HERE, "libmeshlib_amd.so")
 ARCH = os.environ.get("MESHLIB_AMD_ARCH", "gfx950")
 
 SOURCES = ["mi_tree.cpp", "mi_tree_gpu.hip", "mi_kernels.hip", "mi_kernels_keyed.hip", "mi_capi.cpp",
-           "mi_mlp.hip", "mi_mlp_capi.cpp"]
+           "mi_mlp.hip", "mi_mlp_capi.cpp", "mi_noalloc.hip"]
 
 FLAGS = [f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
          "-ffp-contract=off", "-fhip-fp32-correctly-rounded-divide-sqrt",

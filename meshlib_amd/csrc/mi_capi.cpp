@@ -4,6 +4,7 @@
 #include "meshlib_amd.h"
 
 #include "mi_kernels.hpp"
+#include "mi_noalloc.hpp"
 #include "mi_tree.hpp"
 #include "mi_tree_gpu.hpp"
 
@@ -18,6 +19,8 @@
 #include <new>
 #include <string>
 #include <vector>
+
+static_assert(mi::kNoAllocMaxPoints == MI_NOALLOC_MAX_POINTS, "no-allocation capacity of the header");
 
 namespace {
 
@@ -712,6 +715,35 @@ int mi_proj_find_projections_device(mi_mesh* m, const float* d_pts, int64_t n, c
     projSplit(m, m->ws.as<char>() + sortBytes, n, sc);
     const uint32_t* perm = nullptr;
     MI_HIP(mi::sortPointsMorton(d_pts, n, m->sortLo, m->sortHi, m->ws.p, m->ws.bytes, &perm, s));
+    mi::Xf px, tx;
+    if (ptXf) px = toXf(ptXf);
+    if (treeXf) tx = toXf(treeXf);
+    MI_HIP(mi::launchProjections(callView(m), d_pts, perm, n, ptXf ? &px : nullptr, treeXf ? &tx : nullptr, up, lo,
+                                 reinterpret_cast<mi::ProjRec*>(d_res), m->stackD, s, nullptr, nullptr, nullptr, &sc));
+    return MI_OK;
+}
+
+int mi_proj_find_projections_noalloc_device(mi_mesh* m, const float* d_pts, int64_t n, const float* ptXf,
+                                            const float* treeXf, float up, float lo, mi_projection* d_res,
+                                            void* stream)
+{
+    if (!m || n < 0 || (n > 0 && (!d_pts || !d_res))) return fail(MI_E_INVALID, "invalid arguments");
+    if (n > mi::kNoAllocMaxPoints)
+        return fail(MI_E_LIMIT, "more than MI_NOALLOC_MAX_POINTS points in one call of the no-allocation entry point");
+    if (n == 0) return MI_OK;
+    std::lock_guard<std::mutex> lk(m->mu);
+    MI_HIP(hipSetDevice(m->device));
+    hipStream_t s = pick(m, stream);
+    void* scratch = nullptr;
+    MI_HIP(mi::noAllocScratch(m->device, &scratch));
+    // the same workspace layout as mi_proj_find_projections_device, in the fixed scratch
+    const size_t sortBytes = align256(mi::sortWorkspaceBytes(n));
+    if (sortBytes + projSplitBytes(m, n) > mi::kNoAllocScratchBytes)
+        return fail(MI_E_LIMIT, "the no-allocation scratch is too small for this call");
+    mi::SplitScratch sc;
+    projSplit(m, static_cast<char*>(scratch) + sortBytes, n, sc);
+    const uint32_t* perm = nullptr;
+    MI_HIP(mi::sortPointsMorton(d_pts, n, m->sortLo, m->sortHi, scratch, sortBytes, &perm, s));
     mi::Xf px, tx;
     if (ptXf) px = toXf(ptXf);
     if (treeXf) tx = toXf(treeXf);
