@@ -334,8 +334,11 @@ int  mi_batch_run_capped_device(mi_batch* b, const mi_batch_item* items, int32_t
 /* ---------------------------------------------------------------- learned distance bound
  * No counterpart in the reference: a Fourier-feature MLP (99 features -> 4 x 128 SiLU -> softplus)
  * predicts each query's unsigned distance; the prediction caps the search radius at
- * (pred * (1 + margin))^2 and the capped forms above keep the outputs exact.  Trained and evaluated
- * on the GPU with BF16 MFMA kernels accumulating in FP32. */
+ * (pred * (1 + margin) + slack)^2 (slack: 0, or the network's calibrated allowance, mi_mlp_calibrate)
+ * and the capped forms above keep the outputs exact.  Grid forms predict once per 4x4x4 brick, at its
+ * centre c, and cap voxel x at (pred(c) * (1 + margin) + slack + |x - c|)^2: the distance is
+ * 1-Lipschitz, so this bounds x's distance whenever the brick's radius bounds c's.  Trained and
+ * evaluated on the GPU with BF16 MFMA kernels accumulating in FP32. */
 typedef struct mi_mlp mi_mlp;
 /* network for points around `center`, coordinates normalised by `scale` (> 0); seeded Xavier init */
 int  mi_mlp_create(const float center[3], float scale, uint64_t seed, mi_mlp** out);
@@ -352,6 +355,12 @@ int  mi_mlp_train_device(mi_mlp* m, const float* d_pts, const float* d_labels, i
                          int32_t batch, float lr, void* stream);
 /* predicted unsigned distance per point */
 int  mi_mlp_predict_device(mi_mlp* m, const float* d_pts, int64_t n, float* d_dist, void* stream);
+/* sets the network's slack to the `quantile` (0..1) of label - prediction over the n labelled points
+ * (at least 0), so that about a (1 - quantile) share of such queries exceed their cap; *slack = the value.
+ * Synchronises the stream.  mi_mlp_set_slack sets it directly (0 = the uncalibrated cap). */
+int  mi_mlp_calibrate(mi_mlp* m, const float* d_pts, const float* d_labels, int64_t n, float quantile, float* slack,
+                      void* stream);
+int  mi_mlp_set_slack(mi_mlp* m, float slack);
 /* findProjections / calcFromGridWithDistances with the learned cap; exact (see the capped forms) */
 int  mi_proj_find_projections_learned_device(mi_mesh* mesh, mi_mlp* m, float margin, const float* d_pts,
                                              int64_t n, float upDistLimitSq, mi_projection* d_res,
@@ -359,8 +368,8 @@ int  mi_proj_find_projections_learned_device(mi_mesh* mesh, mi_mlp* m, float mar
 int  mi_sdf_grid_learned_device(mi_mesh* mesh, mi_mlp* m, float margin, const int32_t dims[3], const float xf[12],
                                 const mi_dist_options* opts, int32_t zBegin, int32_t zEnd, float* d_res,
                                 int64_t* reruns, void* stream);
-/* the batch (mi_batch_run_capped_device) with caps (pred * (1 + margin))^2 from each item's own
- * network nets[i] (NULL = no cap for that item); exact */
+/* the batch (mi_batch_run_capped_device) with the caps above from each item's own network nets[i]
+ * (NULL = no cap for that item); exact */
 int  mi_batch_run_learned_device(mi_batch* b, const mi_batch_item* items, int32_t count, mi_mlp* const* nets,
                                  float margin, const mi_dist_options* opts, float upDistLimitSq, void* stream,
                                  int64_t* reruns);

@@ -145,6 +145,8 @@ def lib():
         "mi_mlp_gradient_device": (C.c_int, [P, P, P, I64, P, P]),
         "mi_mlp_train_device": (C.c_int, [P, P, P, I64, I32, I32, F, P]),
         "mi_mlp_predict_device": (C.c_int, [P, P, I64, P, P]),
+        "mi_mlp_calibrate": (C.c_int, [P, P, P, I64, F, C.POINTER(F), P]),
+        "mi_mlp_set_slack": (C.c_int, [P, F]),
         "mi_proj_find_projections_learned_device": (C.c_int, [P, P, F, P, I64, F, P, C.POINTER(I64), P]),
         "mi_sdf_grid_learned_device": (C.c_int, [P, P, F, P, P, P, I32, I32, P, C.POINTER(I64), P]),
     }
@@ -520,6 +522,17 @@ class DistanceMLP:
         lab = d_labels.cpu().numpy()
         d = (pred - lab) / self.scale
         return float(0.5 * np.mean(d.astype(np.float64) ** 2))
+
+    def calibrate(self, d_pts, d_labels, quantile: float = 0.99, stream: int = 0) -> float:
+        """Sets the slack added to every capped radius: the `quantile` of (label - prediction) over these
+        labelled device points (float32 [n, 3] and [n]); returns it."""
+        sl = C.c_float()
+        _check(lib().mi_mlp_calibrate(self._h, C.c_void_p(d_pts.data_ptr()), C.c_void_p(d_labels.data_ptr()),
+                                      int(d_labels.shape[0]), float(quantile), C.byref(sl), C.c_void_p(stream or None)))
+        return float(sl.value)
+
+    def set_slack(self, slack: float):
+        _check(lib().mi_mlp_set_slack(self._h, float(slack)))
 
     def train(self, d_pts, d_labels, steps: int, batch: int, lr: float, stream: int = 0):
         _check(lib().mi_mlp_train_device(self._h, C.c_void_p(d_pts.data_ptr()), C.c_void_p(d_labels.data_ptr()),

@@ -786,15 +786,20 @@ int mi_proj_find_projections_capped_device(mi_mesh* m, const float* d_pts, int64
     std::lock_guard<std::mutex> lk(m->mu);
     MI_HIP(hipSetDevice(m->device));
     hipStream_t s = pick(m, stream);
-    // workspace: sort, rerun list, its counter, split scratch
+    // workspace: sort, rerun list, its counter, split scratch.  The split scratch is bound even when the
+    // step budget is 0: the waves with a failed cap hand their queries, uncapped, to the split pass, so no
+    // query is searched again after it (a separate rerun launch over scattered queries ends on its
+    // slowest single walk).  Only a tree too small to cut uses the rerun list.
     const size_t sortBytes = align256(mi::sortWorkspaceBytes(n));
     const size_t listBytes = align256((size_t)n * 4) + 256;
-    if (int rc = ensureWorkspace(m, sortBytes + listBytes + projSplitBytes(m, n))) return rc;
+    const int64_t waves = mi::projectionWaves(n);
+    const size_t splitBytes = mi::splitScratchBytes(waves, 8);
+    if (int rc = ensureWorkspace(m, sortBytes + listBytes + splitBytes)) return rc;
     char* base = m->ws.as<char>();
     uint32_t* list = reinterpret_cast<uint32_t*>(base + sortBytes);
     uint32_t* count = reinterpret_cast<uint32_t*>(base + sortBytes + align256((size_t)n * 4));
     mi::SplitScratch sc;
-    projSplit(m, base + sortBytes + listBytes, n, sc);
+    mi::splitScratchBind(base + sortBytes + listBytes, waves, 8, projBudget(m), sc);
     const uint32_t* perm = nullptr;
     MI_HIP(mi::sortPointsMorton(d_pts, n, m->sortLo, m->sortHi, m->ws.p, sortBytes, &perm, s));
     MI_HIP(mi::fillU32(count, 1, 0u, s));
@@ -803,7 +808,7 @@ int mi_proj_find_projections_capped_device(mi_mesh* m, const float* d_pts, int64
     uint32_t h = 0;
     MI_HIP(hipMemcpyAsync(&h, count, 4, hipMemcpyDeviceToHost, s));
     MI_HIP(hipStreamSynchronize(s));
-    if (h)
+    if (h && m->dev.nCut <= 0)
         MI_HIP(mi::launchProjections(callView(m), d_pts, list, (int64_t)h, nullptr, nullptr, up, 0.f,
                                      reinterpret_cast<mi::ProjRec*>(d_res), m->stackD, s));
     if (reruns) *reruns = h;
@@ -826,16 +831,18 @@ int mi_sdf_grid_capped_device(mi_mesh* m, const int32_t dims[3], const float xf[
     std::lock_guard<std::mutex> lk(m->mu);
     MI_HIP(hipSetDevice(m->device));
     hipStream_t s = pick(m, stream);
-    // workspace: rerun list, its counter, split scratch
+    // workspace: rerun list, its counter, split scratch (bound even with a step budget of 0, as for the
+    // capped projections: failed caps are searched again in the split pass; only a tree too small to cut
+    // uses the rerun list)
     const size_t listBytes = align256(nvox * 4) + 256;
     int32_t budget = 0;
     const int64_t waves = sdfSplitPlan(m, dims, zEnd - zBegin, &budget);
-    const size_t splitBytes = budget > 0 ? mi::splitScratchBytes(waves, sdfLaneBytes(opts)) : 0;
+    const size_t splitBytes = mi::splitScratchBytes(waves, sdfLaneBytes(opts));
     if (int rc = ensureWorkspace(m, listBytes + splitBytes)) return rc;
     uint32_t* list = m->ws.as<uint32_t>();
     uint32_t* count = reinterpret_cast<uint32_t*>(m->ws.as<char>() + align256(nvox * 4));
     mi::SplitScratch sc;
-    if (budget > 0) mi::splitScratchBind(m->ws.as<char>() + listBytes, waves, sdfLaneBytes(opts), budget, sc);
+    mi::splitScratchBind(m->ws.as<char>() + listBytes, waves, sdfLaneBytes(opts), budget, sc);
     mi::DistOpts o{ opts->minDistSq, opts->maxDistSq, opts->nullOutsideMinMax, opts->windingNumberThreshold,
                     opts->windingNumberBeta };
     MI_HIP(mi::fillU32(count, 1, 0u, s));
@@ -844,7 +851,8 @@ int mi_sdf_grid_capped_device(mi_mesh* m, const int32_t dims[3], const float xf[
     uint32_t h = 0;
     MI_HIP(hipMemcpyAsync(&h, count, 4, hipMemcpyDeviceToHost, s));
     MI_HIP(hipStreamSynchronize(s));
-    if (h) MI_HIP(mi::launchSdfVoxels(callView(m), dims, toXf(xf), o, zBegin, list, (int64_t)h, d_res, m->stackD, s));
+    if (h && m->dev.nCut <= 0)
+        MI_HIP(mi::launchSdfVoxels(callView(m), dims, toXf(xf), o, zBegin, list, (int64_t)h, d_res, m->stackD, s));
     if (reruns) *reruns = h;
     return MI_OK;
 }

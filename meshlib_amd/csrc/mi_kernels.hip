@@ -117,7 +117,8 @@ __global__ __launch_bounds__(kBlock) void kSelfIntersections(MeshDev m, int nLog
 // At most 72 VGPRs: seven waves per SIMD (the LDS of seven blocks fits a CU); the walks are bound by
 // the latency of their dependent scalar loads, and the seventh wave hides more of it than the
 // registers it costs (signed distance 105 -> 98 ms at k=8 256^3).
-template <int D, bool FILTER>
+// CAPPED: the capped form (bound and rerun lists given).
+template <int D, bool FILTER, bool CAPPED>
 __global__ __launch_bounds__(kBlock, 7) void kSdfGridCoop(MeshDev m, int dx, int dy, int zBegin, int zEnd,
                                                        int nbx, int nby, int nbz, int nLogical, Xf xf,
                                                        DistOpts o, const float* __restrict__ bound,
@@ -138,22 +139,36 @@ __global__ __launch_bounds__(kBlock, 7) void kSdfGridCoop(MeshDev m, int dx, int
     xfApply(xf, px, py, pz);
     const size_t idx = valid ? ((size_t)(z - zBegin) * dy + y) * dx + x : 0;
     float lim = o.maxDistSq;
-    if (bound && valid) { const float b = bound[idx]; lim = b < lim ? b : lim; }
+    if (CAPPED && valid) { const float b = bound[idx]; lim = b < lim ? b : lim; }
     bool stopped = false;
     const float resSq = coopDistance<D, FILTER>(m, px, py, pz, valid ? lim : -1.f, &stkMem[w],
                                                 sc.list ? sc.budget : 0, &stopped);
-    if (stopped) {
-        // hand the brick to the split pass with every lane's best so far
+    // capped: a lane whose cap (below the plain limit) has found nothing
+    const bool uncapped = CAPPED && valid && !(resSq < lim) && lim < o.maxDistSq;
+    uint64_t failed = 0;
+    if (CAPPED) {
+        failed = waveBallot(uncapped && !stopped);      // searched to the end below a cap too small
+    }
+    if (stopped || (CAPPED && sc.list && failed)) {
+        // hand the brick to the split pass with every lane's best so far.  Capped: a lane with nothing
+        // below its cap continues uncapped from the greedy leaf (a real candidate), so every handed-over
+        // result is exact and nothing is searched a second time after the split pass.
+        float seed = resSq;
+        if (uncapped) {
+            const float g = greedyLeafDistSq(m, px, py, pz);
+            seed = g < o.maxDistSq ? g : o.maxDistSq;
+        }
         uint32_t slot = 0;
         if (lane == 0) slot = atomicAdd(&sc.counters[0], 1u);
         slot = (uint32_t)uniform((int)slot);
         if (lane == 0) sc.list[slot] = (uint32_t)brick;
-        sc.best[(size_t)slot * 64 + lane] = __float_as_uint(valid ? resSq : -1.f);
+        sc.best[(size_t)slot * 64 + lane] = __float_as_uint(valid ? seed : -1.f);
         rankCut(m, valid, px, py, pz, sc.order + (size_t)slot * 64);
         return;
     }
-    const bool again = rerun && valid && !(resSq < lim) && lim < o.maxDistSq;
-    if (rerun) appendWave(again, (uint32_t)idx, rerun, rerunCount);
+    // without a split scratch (a tree too small to cut) the failed lanes go to the rerun list
+    const bool again = CAPPED && uncapped;
+    if (CAPPED) appendWave(again, (uint32_t)idx, rerun, rerunCount);
     sdfStore(m, px, py, pz, valid && !again, resSq, o, reinterpret_cast<uint4*>(&stkMem[w]), out, idx);
 }
 
@@ -192,16 +207,14 @@ __global__ __launch_bounds__(kBlock) void kSdfSplit(MeshDev m, int dx, int dy, i
     }
 }
 
-// Third launch: the handed-over bricks' results (sign from the winding number, NaN band, store).  With
-// caps (bound) and rerun lists, a voxel that found nothing below its cap is appended instead of stored,
-// as in kSdfGridCoop.
+// Third launch: the handed-over bricks' results (sign from the winding number, NaN band, store).  Every
+// handed-over search is exact: the capped form hands its failed caps over uncapped.  With caps (bound),
+// failCount counts the voxels whose cap (below maxDistSq) was below their distance.
 template <int D>
 __global__ __launch_bounds__(kBlock) void kSdfSplitFinish(MeshDev m, int dx, int dy, int zBegin, int zEnd,
                                                             int nbx, int nby, int nbz, Xf xf, DistOpts o,
                                                             SplitScratch sc, const float* __restrict__ bound,
-                                                            uint32_t* __restrict__ rerun,
-                                                            uint32_t* __restrict__ rerunCount,
-                                                            float* __restrict__ out)
+                                                            uint32_t* __restrict__ failCount, float* __restrict__ out)
 {
     __shared__ float4 stkMem[kWaves][2 * quadStack(D)];
     const int lane = threadIdx.x & 63;
@@ -219,11 +232,12 @@ __global__ __launch_bounds__(kBlock) void kSdfSplitFinish(MeshDev m, int dx, int
         xfApply(xf, px, py, pz);
         const size_t idx = valid ? ((size_t)(z - zBegin) * dy + y) * dx + x : 0;
         const float resSq = __uint_as_float(sc.best[(size_t)t * 64 + lane]);
-        float lim = o.maxDistSq;
-        if (bound && valid) { const float b = bound[idx]; lim = b < lim ? b : lim; }
-        const bool again = rerun && valid && !(resSq < lim) && lim < o.maxDistSq;
-        if (rerun) appendWave(again, (uint32_t)idx, rerun, rerunCount);
-        sdfStore(m, px, py, pz, valid && !again, resSq, o, reinterpret_cast<uint4*>(stkMem[w]), out, idx);
+        if (bound) {
+            const float b = valid ? bound[idx] : o.maxDistSq;
+            const uint64_t f = waveBallot(valid && b < o.maxDistSq && !(resSq < b));
+            if (f && lane == 0) atomicAdd(failCount, (uint32_t)__popcll(f));
+        }
+        sdfStore(m, px, py, pz, valid, resSq, o, reinterpret_cast<uint4*>(stkMem[w]), out, idx);
     }
 }
 
@@ -685,7 +699,7 @@ hipError_t launchSdfGrid(const MeshDev& m, const int32_t dims[3], const Xf& xf, 
     if (o.minDistSq > 0 && !bound && m.traversal == (int)Traversal::WaveCoop)
         return launchSdfGridKeyed(m, dims, xf, o, zBegin, zEnd, out, stackD, s, split);
     const bool coop = (m.traversal == (int)Traversal::WaveCoop || bound) && !(o.minDistSq > 0);
-    if (bound && !coop) return hipErrorInvalidValue;
+    if (bound && (!coop || !rerun || !rerunCount)) return hipErrorInvalidValue;
     // The leaf filter saves instructions but adds a dependent load to every leaf visit.  It pays unless
     // a deep tree meets a small grid: then few waves walk long chains and nothing hides that load.
     // Measured on / off (bumpy spheres): k=5 64^3 0.89, k=7 128^3 0.87, k=8 96^3 0.96, k=8 256^3 0.87,
@@ -695,7 +709,8 @@ hipError_t launchSdfGrid(const MeshDev& m, const int32_t dims[3], const Xf& xf, 
     const int64_t waves = sdfGridWaves(dims, zBegin, zEnd);
     const bool filter = filterMode < 0 ? waves > ((int64_t)1 << (m.depth > 8 ? m.depth - 8 : 0)) : filterMode != 0;
     SplitScratch sc;
-    if (coop && split && split->list && split->budget > 0 && m.nCut > 0) {
+    // the capped form takes the split scratch even with no step budget (its failed waves use it)
+    if (coop && split && split->list && (split->budget > 0 || bound) && m.nCut > 0) {
         if ((int64_t)split->cap < sdfGridWaves(dims, zBegin, zEnd)) return hipErrorInvalidValue;
         sc = *split;
         hipError_t e = fillU32(sc.counters, 3, 0u, s);
@@ -704,7 +719,8 @@ hipError_t launchSdfGrid(const MeshDev& m, const int32_t dims[3], const Xf& xf, 
     return dispatchDepth(stackD, [&](auto Dc) {
         constexpr int D = decltype(Dc)::value;
         if (coop) {
-            const auto gridKernel = filter ? kSdfGridCoop<D, true> : kSdfGridCoop<D, false>;
+            const auto gridKernel = bound ? (filter ? kSdfGridCoop<D, true, true> : kSdfGridCoop<D, false, true>)
+                                          : (filter ? kSdfGridCoop<D, true, false> : kSdfGridCoop<D, false, false>);
             const auto splitKernel = filter ? kSdfSplit<D, true> : kSdfSplit<D, false>;
             hipLaunchKernelGGL(gridKernel, dim3(chunkedGrid(nLogical)), dim3(kBlock), 0, s, m, dims[0], dims[1],
                                zBegin, zEnd, nbx, nby, nbz, nLogical, xf, o, bound, rerun, rerunCount, sc, out);
@@ -713,7 +729,7 @@ hipError_t launchSdfGrid(const MeshDev& m, const int32_t dims[3], const Xf& xf, 
                 hipLaunchKernelGGL(splitKernel, dim3(blocks), dim3(kBlock), 0, s, statLaunch(m, 1), dims[0], dims[1], zBegin, zEnd,
                                    nbx, nby, nbz, xf, sc);
                 hipLaunchKernelGGL(kSdfSplitFinish<D>, dim3(blocks), dim3(kBlock), 0, s, statLaunch(m, 2), dims[0], dims[1],
-                                   zBegin, zEnd, nbx, nby, nbz, xf, o, sc, bound, rerun, rerunCount, out);
+                                   zBegin, zEnd, nbx, nby, nbz, xf, o, sc, bound, rerunCount, out);
             }
         } else {
             hipLaunchKernelGGL(kSdfGrid<D>, dim3(chunkedGrid(nLogical)), dim3(kBlock), 0, s, m, dims[0],

@@ -114,8 +114,9 @@ __device__ __forceinline__ int keyLeaf(const MeshDev& m, uint32_t key, float px,
 // bound (optional, indexed like the output): a per-query cap on upDistLimitSq.  With rerun lists
 // given, a query whose cap was below `up` and that found nothing below the cap is not written but
 // appended to the list, to be searched again without the cap (any cap above the true distance
-// gives the reference's record, so the combination is exact).  lo: loDistLimitSq.
-template <int D>
+// gives the reference's record, so the combination is exact).  lo: loDistLimitSq.  CAPPED: rerun lists
+// given (the capped form); without them a bound is a plain per-query limit (the bounded form).
+template <int D, bool CAPPED>
 __global__ __launch_bounds__(kBlock) void kProjectionsCoop(MeshDev m, const float* __restrict__ pts,
                                                            const uint32_t* __restrict__ perm, int64_t n,
                                                            int nLogical, Xf ptXf, int hasPtXf, float up, float lo,
@@ -143,12 +144,31 @@ __global__ __launch_bounds__(kBlock) void kProjectionsCoop(MeshDev m, const floa
     uint32_t key = 0;
     bool stopped = false;
     coopProject<D>(m, px, py, pz, lim, valid, &stkMem[w], h, &key, sc.list ? sc.budget : 0, &stopped, lo);
-    if (stopped) {
-        handOver(m, sc, (uint32_t)(lb * kWaves + w), valid, px, py, pz, h.distSq, key);
+    // capped: a lane whose cap (below up) has found nothing
+    const bool uncapped = CAPPED && valid && h.face < 0 && lim < up;
+    uint64_t failed = 0;
+    if (CAPPED) {
+        failed = waveBallot(uncapped && !stopped);      // searched to the end below a cap too small
+    }
+    if (stopped || (CAPPED && sc.list && failed)) {
+        // capped: a lane with nothing below its cap continues uncapped, from a radius just above the
+        // greedy leaf as coopProject starts, so every handed-over record is exact and nothing is searched
+        // a second time after the split pass
+        float seedE = h.distSq;
+        uint32_t seedKey = key;
+        if (uncapped) {
+            const float loE = effectiveLo(lo, up);
+            const float g = loOrder(greedyLeafDistSq(m, px, py, pz), loE);
+            const float above = g < FLT_MIN ? FLT_MIN : __uint_as_float(__float_as_uint(g) + 1u);
+            seedE = g < up && above < up ? above : up;
+            seedKey = 0;
+        }
+        handOver(m, sc, (uint32_t)(lb * kWaves + w), valid, px, py, pz, seedE, seedKey);
         return;
     }
-    const bool again = rerun && valid && h.face < 0 && lim < up;
-    if (rerun) appendWave(again, (uint32_t)q, rerun, rerunCount);
+    // without a split scratch (a tree too small to cut) the failed lanes go to the rerun list
+    const bool again = uncapped;
+    if (CAPPED) appendWave(again, (uint32_t)q, rerun, rerunCount);
     if (!valid || again) return;
     float4* o = reinterpret_cast<float4*>(out + q);
     o[0] = make_float4(__int_as_float(h.face), h.px, h.py, h.pz);
@@ -212,14 +232,12 @@ __global__ __launch_bounds__(kBlock) void kKeyedSplit(MeshDev m, Src src, SplitS
 }
 
 // Third launch of the split projections: the handed-over waves' records, from the leaf the winning
-// key names (its closest point and true squared distance recomputed exactly as the leaf test).  With
-// caps (src.bound) and rerun lists, a query that found nothing below its cap is appended instead of
-// written, as in kProjectionsCoop.
+// key names (its closest point and true squared distance recomputed exactly as the leaf test).  caps
+// (the capped form): failCount counts the queries whose cap (below up) was below their result.
 template <int D>
 __global__ __launch_bounds__(kBlock) void kProjSplitFinish(MeshDev m, PointSource src, SplitScratch sc,
-                                                           uint32_t* __restrict__ rerun,
-                                                           uint32_t* __restrict__ rerunCount,
-                                                           ProjRec* __restrict__ out)
+                                                           const float* __restrict__ caps,
+                                                           uint32_t* __restrict__ failCount, ProjRec* __restrict__ out)
 {
     const int lane = threadIdx.x & 63;
     const unsigned long long* best = reinterpret_cast<const unsigned long long*>(sc.best);
@@ -234,13 +252,15 @@ __global__ __launch_bounds__(kBlock) void kProjSplitFinish(MeshDev m, PointSourc
         const bool valid = src.get(sc.list[t], lane, q, px, py, pz, lim);
         const unsigned long long b = valid ? best[(size_t)t * 64 + lane] : 0ull;
         const float e = __uint_as_float((uint32_t)(b >> 32));
-        // a winner has d < lim, i.e. order value < lim (loE < lim); otherwise the reference's empty result,
-        // or (a cap below up) a rerun without the cap
-        const int leaf = valid && e < lim ? keyLeaf(m, (uint32_t)b, px, py, pz) : -1;
-        const bool again = rerun && valid && leaf < 0 && lim < src.up;
-        if (rerun) appendWave(again, (uint32_t)q, rerun, rerunCount);
-        if (!valid || again) continue;
+        if (caps) {
+            const float c = valid ? caps[q] : src.up;
+            const uint64_t f = waveBallot(valid && c < src.up && !(e < c));
+            if (f && lane == 0) atomicAdd(failCount, (uint32_t)__popcll(f));
+        }
+        if (!valid) continue;
         float4* o = reinterpret_cast<float4*>(out + q);
+        // a winner has d < lim, i.e. order value < lim (loE < lim); otherwise the reference's empty result
+        const int leaf = e < lim ? keyLeaf(m, (uint32_t)b, px, py, pz) : -1;
         if (leaf < 0) {
             o[0] = make_float4(__int_as_float(-1), 0.f, 0.f, 0.f);
             o[1] = make_float4(__int_as_float(-1), 0.f, 0.f, lim);
@@ -337,9 +357,10 @@ hipError_t launchProjections(const MeshDev& m, const float* pts, const uint32_t*
     // transform (applied per box) keeps the per-lane form
     const bool coopOk = !treeXf;
     const bool coop = coopOk && (m.traversal == (int)Traversal::WaveCoop || rerun);
-    if (rerun && !coop) return hipErrorInvalidValue;
+    if (rerun && (!coop || !rerunCount || !bound)) return hipErrorInvalidValue;
     SplitScratch sc;
-    if (coop && split && split->list && split->budget > 0 && m.nCut > 0) {
+    // the capped form takes the split scratch even with no step budget (its failed waves use it)
+    if (coop && split && split->list && (split->budget > 0 || rerun) && m.nCut > 0) {
         if ((int64_t)split->cap < projectionWaves(n)) return hipErrorInvalidValue;
         sc = *split;
         hipError_t e = fillU32(sc.counters, 3, 0u, s);
@@ -348,15 +369,21 @@ hipError_t launchProjections(const MeshDev& m, const float* pts, const uint32_t*
     return dispatchDepth(stackD, [&](auto Dc) {
         constexpr int D = decltype(Dc)::value;
         if (coop) {
-            hipLaunchKernelGGL(kProjectionsCoop<D>, dim3(paddedGrid(nLogical)), dim3(kBlock), 0, s, m, pts,
-                               perm, n, nLogical, px, ptXf ? 1 : 0, up, lo, bound, rerun, rerunCount, sc, out);
+            if (rerun)
+                hipLaunchKernelGGL((kProjectionsCoop<D, true>), dim3(paddedGrid(nLogical)), dim3(kBlock), 0, s, m,
+                                   pts, perm, n, nLogical, px, ptXf ? 1 : 0, up, lo, bound, rerun, rerunCount, sc, out);
+            else
+                hipLaunchKernelGGL((kProjectionsCoop<D, false>), dim3(paddedGrid(nLogical)), dim3(kBlock), 0, s, m,
+                                   pts, perm, n, nLogical, px, ptXf ? 1 : 0, up, lo, bound, rerun, rerunCount, sc, out);
             if (sc.list) {
-                const PointSource src{ pts, perm, n, px, ptXf ? 1 : 0, up, lo, bound };
+                // the capped form hands over uncapped searches (the bound no longer applies); the bounded
+                // form keeps its per-query limits
+                const PointSource src{ pts, perm, n, px, ptXf ? 1 : 0, up, lo, rerun ? nullptr : bound };
                 const int blocks = persistentBlocks();
                 hipLaunchKernelGGL((kKeyedSplit<D, PointSource>), dim3(blocks), dim3(kBlock), 0, s, statLaunch(m, 1),
                                    src, sc);
                 hipLaunchKernelGGL(kProjSplitFinish<D>, dim3(blocks), dim3(kBlock), 0, s, statLaunch(m, 2), src, sc,
-                                   rerun, rerunCount, out);
+                                   rerun ? bound : nullptr, rerunCount, out);
             }
         } else if (treeXf)
             hipLaunchKernelGGL((kProjections<true, D>), dim3(paddedGrid(nLogical)), dim3(kBlock), 0, s,

@@ -122,12 +122,10 @@ __device__ __forceinline__ void hiddenLayers(const Params& p, uint16_t* Ws, uint
     __syncthreads();
 }
 
-// capMargin >= 0: write the capped search radius (pred * (1 + margin))^2 instead of the distance
-template <bool TRAIN>
-__global__ __launch_bounds__(256) void kForward(Params p, const float* __restrict__ pts,
-                                                const uint32_t* __restrict__ perm, int64_t n,
-                                                float* __restrict__ out, float capMargin, float* __restrict__ Zs,
-                                                uint16_t* __restrict__ Hs, float* __restrict__ U)
+// training forward: keeps each layer's pre-activations Z and BF16 activations H, and the output's U
+__global__ __launch_bounds__(256) void kForwardTrain(Params p, const float* __restrict__ pts, int64_t n,
+                                                     float* __restrict__ Zs, uint16_t* __restrict__ Hs,
+                                                     float* __restrict__ U)
 {
     __shared__ __attribute__((aligned(16))) uint16_t Ws[kHidden * LDP];
     __shared__ __attribute__((aligned(16))) uint16_t Act[kTileRows * LDP];
@@ -137,16 +135,13 @@ __global__ __launch_bounds__(256) void kForward(Params p, const float* __restric
         const int r = tid >> 2, part = tid & 3;
         const int64_t gi = row0 + r;
         float x[3] = { 0.f, 0.f, 0.f };
-        if (gi < n) {
-            const int64_t q = perm ? (int64_t)perm[gi] : gi;
-            normalised(p, pts + 3 * q, x);
-        }
+        if (gi < n) normalised(p, pts + 3 * gi, x);
         for (int j = 0; j < 32; ++j) {
             const int fi = 32 * part + j;
             Act[r * LDP + fi] = f2bf(feature(fi, x, p.piFreq));
         }
     }
-    hiddenLayers<TRAIN>(p, Ws, Act, row0, n, Zs, Hs);
+    hiddenLayers<true>(p, Ws, Act, row0, n, Zs, Hs);
     // output layer: u = sum_c h4[c] * w5[c] + b5 (four partial sums of 32, combined pairwise)
     const int r = tid >> 2, part = tid & 3;
     float s = 0.f;
@@ -158,18 +153,7 @@ __global__ __launch_bounds__(256) void kForward(Params p, const float* __restric
     s += __shfl_xor(s, 2);
     const int64_t gi = row0 + r;
     if (part != 0 || gi >= n) return;
-    const float u = s + p.theta[kOffB5];
-    if (TRAIN) U[gi] = u;
-    if (out) {
-        const int64_t q = perm ? (int64_t)perm[gi] : gi;
-        const float d = softplusf(u) * p.scale;
-        if (capMargin >= 0.f) {
-            const float c = d * (1.f + capMargin);
-            out[q] = c * c;
-        } else {
-            out[q] = d;
-        }
-    }
+    U[gi] = s + p.theta[kOffB5];
 }
 
 // loss = 1/(2B) * sum (pred/scale - label/scale)^2 over the batch; per-tile partial gradients
@@ -339,10 +323,13 @@ constexpr int kPredGroups = 4;                        // 16-point groups per wav
 constexpr int kPredPoints = 16 * kPredGroups;         // points per wave and tile
 constexpr int kFragBlocks = kLayers * 8 * 4;          // (layer, tile, chunk) fragment blocks
 
-// query source of the inference: free points, or the voxel centres of a grid slab (x fastest)
+struct Xf3 { float m[9]; };   // linear part of a grid transform (row-major)
+
+// query source of the inference: free points, or the centres of the 4x4x4 bricks of a grid slab (brick
+// index (bz nby + by) nbx + bx, centre at grid coordinates 4 b + 1.5, z counted from zBegin)
 struct PredSource {
-    const float* pts;      // null: grid
-    int dx, dy, zBegin;
+    const float* pts;      // null: brick centres
+    int nbx, nby, zBegin;
     float xf[12];
     __device__ __forceinline__ void point(int64_t i, float p[3]) const
     {
@@ -350,10 +337,10 @@ struct PredSource {
             p[0] = pts[3 * i]; p[1] = pts[3 * i + 1]; p[2] = pts[3 * i + 2];
             return;
         }
-        const int64_t layer = (int64_t)dx * dy;
-        const float z = (float)(zBegin + (int)(i / layer));
+        const int64_t layer = (int64_t)nbx * nby;
+        const float z = (float)zBegin + 4.f * (float)(int)(i / layer) + 1.5f;
         const int rem = (int)(i % layer);
-        const float x = (float)(rem % dx), y = (float)(rem / dx);
+        const float x = 4.f * (float)(rem % nbx) + 1.5f, y = 4.f * (float)(rem / nbx) + 1.5f;
         p[0] = (xf[0] * x + xf[1] * y + xf[2] * z) + xf[9];
         p[1] = (xf[3] * x + xf[4] * y + xf[5] * z) + xf[10];
         p[2] = (xf[6] * x + xf[7] * y + xf[8] * z) + xf[11];
@@ -374,8 +361,10 @@ __device__ __forceinline__ float featureFast(int f, const float x[3], const floa
 
 __device__ __forceinline__ uint32_t pack2(uint16_t lo, uint16_t hi) { return (uint32_t)lo | ((uint32_t)hi << 16); }
 
+// out[i]: the predicted distance (capMargin < 0), else the cap radius r = pred (1 + capMargin) + slack, squared
+// when capSquared
 __global__ __launch_bounds__(64 * kPredWaves) void kPredict(Params p, PredSource src, int64_t n,
-                                                             float* __restrict__ out, float capMargin)
+                                                             float* __restrict__ out, float capMargin, int capSquared)
 {
     __shared__ __attribute__((aligned(16))) uint4 Wf[kFragBlocks * 64];
     __shared__ float bias[kLayers * kHidden];
@@ -491,14 +480,33 @@ __global__ __launch_bounds__(64 * kPredWaves) void kPredict(Params p, PredSource
             if (lg == 0 && i < n) {
                 const float d = softplusf(s + b5) * p.scale;
                 if (capMargin >= 0.f) {
-                    const float cap = d * (1.f + capMargin);
-                    out[i] = cap * cap;
+                    const float r = d * (1.f + capMargin) + p.slack;
+                    out[i] = capSquared ? r * r : r;
                 } else {
                     out[i] = d;
                 }
             }
         }
     }
+}
+
+// squared caps of the slab's voxels from their bricks' radii: (r_b + |x - c_b|)^2
+__global__ void kBrickCaps(int dx, int dy, int64_t count, int nbx, int nby, Xf3 A, const float* __restrict__ radius,
+                           float* __restrict__ capSq)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const int64_t layer = (int64_t)dx * dy;
+    const int z = (int)(i / layer), rem = (int)(i % layer);
+    const int x = rem % dx, y = rem / dx;
+    const int64_t b = ((int64_t)(z >> 2) * nby + (y >> 2)) * nbx + (x >> 2);
+    // offset from the brick centre in grid steps, then in space
+    const float gx = (float)(x & 3) - 1.5f, gy = (float)(y & 3) - 1.5f, gz = (float)(z & 3) - 1.5f;
+    const float wx = A.m[0] * gx + A.m[1] * gy + A.m[2] * gz;
+    const float wy = A.m[3] * gx + A.m[4] * gy + A.m[5] * gz;
+    const float wz = A.m[6] * gx + A.m[7] * gy + A.m[8] * gz;
+    const float r = radius[b] + sqrtf(wx * wx + wy * wy + wz * wz);
+    capSq[i] = r * r;
 }
 
 __global__ void kReduce(const float* __restrict__ partial, int nBlocks, float* __restrict__ grad)
@@ -564,34 +572,42 @@ int predictBlocks(int64_t n)
 }
 } // namespace
 
-hipError_t forward(const Params& p, const float* pts, const uint32_t* perm, int64_t n, float* out, float capMargin,
-                   hipStream_t s)
+hipError_t forward(const Params& p, const float* pts, int64_t n, float* out, float capMargin, hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
-    if (perm) {
-        const int blocks = (int)((n + kTileRows - 1) / kTileRows);
-        hipLaunchKernelGGL(kForward<false>, dim3(blocks), dim3(256), 0, s, p, pts, perm, n, out, capMargin, nullptr,
-                           nullptr, nullptr);
-        return hipGetLastError();
-    }
     PredSource src{};
     src.pts = pts;
-    hipLaunchKernelGGL(kPredict, dim3(predictBlocks(n)), dim3(64 * kPredWaves), 0, s, p, src, n, out, capMargin);
+    hipLaunchKernelGGL(kPredict, dim3(predictBlocks(n)), dim3(64 * kPredWaves), 0, s, p, src, n, out, capMargin, 1);
     return hipGetLastError();
 }
 
-hipError_t forwardGrid(const Params& p, const int32_t dims[3], const float xf[12], int32_t zBegin, int32_t zEnd,
-                       float* out, float capMargin, hipStream_t s)
+int64_t gridCapBricks(const int32_t dims[3], int32_t zBegin, int32_t zEnd)
 {
-    const int64_t n = (int64_t)dims[0] * dims[1] * (zEnd - zBegin);
-    if (n <= 0) return hipSuccess;
+    if (dims[0] <= 0 || dims[1] <= 0 || zEnd <= zBegin) return 0;
+    return (int64_t)((dims[0] + 3) / 4) * ((dims[1] + 3) / 4) * ((zEnd - zBegin + 3) / 4);
+}
+
+hipError_t gridCaps(const Params& p, const int32_t dims[3], const float xf[12], int32_t zBegin, int32_t zEnd,
+                    float capMargin, float* radius, float* capSq, hipStream_t s)
+{
+    const int64_t bricks = gridCapBricks(dims, zBegin, zEnd);
+    if (bricks <= 0) return hipSuccess;
+    if (!(capMargin >= 0.f)) return hipErrorInvalidValue;
     PredSource src{};
     src.pts = nullptr;
-    src.dx = dims[0];
-    src.dy = dims[1];
+    src.nbx = (dims[0] + 3) / 4;
+    src.nby = (dims[1] + 3) / 4;
     src.zBegin = zBegin;
     for (int i = 0; i < 12; ++i) src.xf[i] = xf[i];
-    hipLaunchKernelGGL(kPredict, dim3(predictBlocks(n)), dim3(64 * kPredWaves), 0, s, p, src, n, out, capMargin);
+    hipLaunchKernelGGL(kPredict, dim3(predictBlocks(bricks)), dim3(64 * kPredWaves), 0, s, p, src, bricks, radius,
+                       capMargin, 0);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const int64_t count = (int64_t)dims[0] * dims[1] * (zEnd - zBegin);
+    Xf3 A;
+    for (int i = 0; i < 9; ++i) A.m[i] = xf[i];
+    hipLaunchKernelGGL(kBrickCaps, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, dims[0], dims[1], count,
+                       src.nbx, src.nby, A, radius, capSq);
     return hipGetLastError();
 }
 
@@ -600,8 +616,7 @@ hipError_t gradient(const Params& p, const float* pts, const float* labels, int6
 {
     if (n <= 0) return hipSuccess;
     const int blocks = (int)((n + kTileRows - 1) / kTileRows);
-    hipLaunchKernelGGL(kForward<true>, dim3(blocks), dim3(256), 0, s, p, pts, nullptr, n, nullptr, -1.f, tb.Z, tb.H,
-                       tb.U);
+    hipLaunchKernelGGL(kForwardTrain, dim3(blocks), dim3(256), 0, s, p, pts, n, tb.Z, tb.H, tb.U);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(kBackward, dim3(blocks), dim3(256), 0, s, p, pts, labels, n, 1.f / (float)n, tb.Z, tb.H,

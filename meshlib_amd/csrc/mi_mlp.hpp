@@ -34,6 +34,7 @@ struct Params {
     float center[3];
     float invScale, scale;
     float piFreq[kBands];    // pi * 2^(k/3)
+    float slack;             // calibrated allowance added to every capped radius (0 until calibrated)
 };
 
 struct TrainBuffers {
@@ -43,14 +44,17 @@ struct TrainBuffers {
     float* partial;    // [ceil(n / kTileRows)][kParamsPadded] per-tile gradients
 };
 
-// predicted distance (capMargin < 0) or capped squared search radius (pred (1 + capMargin))^2 per point,
-// written at the point's index (perm optional)
-hipError_t forward(const Params& p, const float* pts, const uint32_t* perm, int64_t n, float* out, float capMargin,
-                   hipStream_t s);
-// the same over the voxel centres of the z-slab [zBegin, zEnd) of a grid (x fastest), without
-// materialising the points
-hipError_t forwardGrid(const Params& p, const int32_t dims[3], const float xf[12], int32_t zBegin, int32_t zEnd,
-                       float* out, float capMargin, hipStream_t s);
+// predicted distance (capMargin < 0) or capped squared search radius (pred (1 + capMargin) + slack)^2 per
+// point, written at the point's index
+hipError_t forward(const Params& p, const float* pts, int64_t n, float* out, float capMargin, hipStream_t s);
+// Squared caps of the voxels of the z-slab [zBegin, zEnd) of a grid (x fastest): one prediction per 4x4x4
+// brick at its centre c, r = pred(c) (1 + capMargin) + slack, and for voxel x the cap (r + |x - c|)^2 -- an
+// upper bound of x's distance whenever r bounds c's (distance is 1-Lipschitz).  radius: scratch of one
+// float per brick.
+hipError_t gridCaps(const Params& p, const int32_t dims[3], const float xf[12], int32_t zBegin, int32_t zEnd,
+                    float capMargin, float* radius, float* capSq, hipStream_t s);
+// bricks of the slab (scratch size of gridCaps' radius, in floats)
+int64_t gridCapBricks(const int32_t dims[3], int32_t zBegin, int32_t zEnd);
 // gradient of 1/(2n) sum (pred/scale - label/scale)^2 over the n points into grad[kParamsPadded]
 hipError_t gradient(const Params& p, const float* pts, const float* labels, int64_t n, const TrainBuffers& tb,
                     float* grad, hipStream_t s);

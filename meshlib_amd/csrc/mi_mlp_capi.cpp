@@ -97,6 +97,7 @@ hipError_t setupParams(mi_mlp* m, const float center[3], float scale)
     p.scale = scale;
     p.invScale = 1.f / scale;
     for (int k = 0; k < kBands; ++k) p.piFreq[k] = (float)(M_PI * std::pow(2.0, k / 3.0));
+    p.slack = 0.f;
     return hipSuccess;
 }
 
@@ -220,7 +221,43 @@ int mi_mlp_predict_device(mi_mlp* m, const float* d_pts, int64_t n, float* d_dis
     if (!m || n < 0 || (n > 0 && (!d_pts || !d_dist))) return mi_internal_fail(MI_E_INVALID, "invalid arguments");
     std::lock_guard<std::mutex> lk(m->mu);
     MLP_HIP(hipSetDevice(m->device));
-    MLP_HIP(mi::mlp::forward(m->params, d_pts, nullptr, n, d_dist, -1.f, static_cast<hipStream_t>(stream)));
+    MLP_HIP(mi::mlp::forward(m->params, d_pts, n, d_dist, -1.f, static_cast<hipStream_t>(stream)));
+    return MI_OK;
+}
+
+int mi_mlp_calibrate(mi_mlp* m, const float* d_pts, const float* d_labels, int64_t n, float quantile, float* slack,
+                     void* stream)
+{
+    if (!m || n <= 0 || !d_pts || !d_labels || !(quantile >= 0.f && quantile <= 1.f))
+        return mi_internal_fail(MI_E_INVALID, "invalid arguments");
+    if (n > (int64_t)1 << 26) return mi_internal_fail(MI_E_LIMIT, "more than 2^26 calibration samples");
+    std::lock_guard<std::mutex> lk(m->mu);
+    MLP_HIP(hipSetDevice(m->device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    MLP_HIP(m->work.ensure((size_t)n * 4));
+    MLP_HIP(mi::mlp::forward(m->params, d_pts, n, m->work.as<float>(), -1.f, s));
+    std::vector<float> pred((size_t)n), lab((size_t)n);
+    MLP_HIP(hipMemcpyAsync(pred.data(), m->work.p, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+    MLP_HIP(hipMemcpyAsync(lab.data(), d_labels, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+    MLP_HIP(hipStreamSynchronize(s));
+    // how far the true distance exceeds the prediction, at the requested quantile (never below 0)
+    std::vector<float> under((size_t)n);
+    for (int64_t i = 0; i < n; ++i) {
+        const float u = lab[(size_t)i] - pred[(size_t)i];
+        under[(size_t)i] = u == u ? u : 0.f;
+    }
+    const size_t k = std::min((size_t)n - 1, (size_t)std::floor((double)quantile * (double)(n - 1) + 0.5));
+    std::nth_element(under.begin(), under.begin() + (ptrdiff_t)k, under.end());
+    m->params.slack = std::max(0.f, under[k]);
+    if (slack) *slack = m->params.slack;
+    return MI_OK;
+}
+
+int mi_mlp_set_slack(mi_mlp* m, float slack)
+{
+    if (!m || !(slack >= 0.f)) return mi_internal_fail(MI_E_INVALID, "invalid arguments");
+    std::lock_guard<std::mutex> lk(m->mu);
+    m->params.slack = slack;
     return MI_OK;
 }
 
@@ -236,8 +273,7 @@ int mi_proj_find_projections_learned_device(mi_mesh* mesh, mi_mlp* m, float marg
     std::lock_guard<std::mutex> lk(m->mu);
     MLP_HIP(hipSetDevice(m->device));
     MLP_HIP(m->work.ensure((size_t)n * 4));
-    MLP_HIP(mi::mlp::forward(m->params, d_pts, nullptr, n, m->work.as<float>(), margin,
-                             static_cast<hipStream_t>(stream)));
+    MLP_HIP(mi::mlp::forward(m->params, d_pts, n, m->work.as<float>(), margin, static_cast<hipStream_t>(stream)));
     return mi_proj_find_projections_capped_device(mesh, d_pts, n, m->work.as<float>(), upDistLimitSq, d_res, reruns,
                                                   stream);
 }
@@ -255,10 +291,12 @@ int mi_sdf_grid_learned_device(mi_mesh* mesh, mi_mlp* m, float margin, const int
     // the network's lock is held through the capped search, which reads the caps from its workspace
     std::lock_guard<std::mutex> lk(m->mu);
     MLP_HIP(hipSetDevice(m->device));
-    MLP_HIP(m->work.ensure(align256(nvox * 4)));
+    const size_t capBytes = align256(nvox * 4);
+    MLP_HIP(m->work.ensure(capBytes + (size_t)mi::mlp::gridCapBricks(dims, zBegin, zEnd) * 4));
     float* caps = m->work.as<float>();
+    float* radius = reinterpret_cast<float*>(m->work.as<char>() + capBytes);
     hipStream_t s = static_cast<hipStream_t>(stream);
-    MLP_HIP(mi::mlp::forwardGrid(m->params, dims, xf, zBegin, zEnd, caps, margin, s));
+    MLP_HIP(mi::mlp::gridCaps(m->params, dims, xf, zBegin, zEnd, margin, radius, caps, s));
     return mi_sdf_grid_capped_device(mesh, dims, xf, opts, m->work.as<float>(), zBegin, zEnd, d_res, reruns, stream);
 }
 
@@ -272,17 +310,19 @@ int mi_batch_run_learned_device(mi_batch* b, const mi_batch_item* items, int32_t
     if (count == 0) return MI_OK;
     hipStream_t s = static_cast<hipStream_t>(stream);
     // caps of every item (squared, from its own network), then the capped two-pass batch
-    size_t need = 0;
+    size_t need = 0, radiusBytes = 0;
     for (int32_t i = 0; i < count; ++i) {
         const mi_batch_item& it = items[i];
         if (it.dims[0] < 0 || it.dims[1] < 0 || it.dims[2] < 0 || it.numPts < 0)
             return mi_internal_fail(MI_E_INVALID, "invalid batch item");
         const size_t nvox = (size_t)it.dims[0] * it.dims[1] * it.dims[2];
         need += align256(nvox * 4) + align256((size_t)it.numPts * 4);
+        radiusBytes += align256((size_t)mi::mlp::gridCapBricks(it.dims, 0, it.dims[2]) * 4);
     }
     Buf caps;
-    MLP_HIP(caps.ensure(need + 256));
+    MLP_HIP(caps.ensure(need + radiusBytes + 256));
     char* base = caps.as<char>();
+    char* radiusBase = base + need;
     std::vector<const float*> capSdf((size_t)count, nullptr), capProj((size_t)count, nullptr);
     size_t off = 0;
     for (int32_t i = 0; i < count; ++i) {
@@ -296,12 +336,14 @@ int mi_batch_run_learned_device(mi_batch* b, const mi_batch_item* items, int32_t
         if (!net) continue;
         std::lock_guard<std::mutex> lk(net->mu);
         MLP_HIP(hipSetDevice(net->device));
+        float* radius = reinterpret_cast<float*>(radiusBase);
+        radiusBase += align256((size_t)mi::mlp::gridCapBricks(it.dims, 0, it.dims[2]) * 4);
         if (nvox) {
-            MLP_HIP(mi::mlp::forwardGrid(net->params, it.dims, it.xf, 0, it.dims[2], cs, margin, s));
+            MLP_HIP(mi::mlp::gridCaps(net->params, it.dims, it.xf, 0, it.dims[2], margin, radius, cs, s));
             capSdf[i] = cs;
         }
         if (it.numPts) {
-            MLP_HIP(mi::mlp::forward(net->params, it.d_pts, nullptr, it.numPts, cp, margin, s));
+            MLP_HIP(mi::mlp::forward(net->params, it.d_pts, it.numPts, cp, margin, s));
             capProj[i] = cp;
         }
     }

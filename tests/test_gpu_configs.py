@@ -2,9 +2,10 @@
 
 * Learned bound (k=8 bumpy sphere, 1,310,720 triangles): a 2,000-step Adam run of batch 8,192 on 2^20
   labels from the GPU projector, then 2^22 projections and a 256^3 signed-distance volume with the
-  capped search at m = 0.05, 0.1, 0.25.  Every learned output equals the plain one bit-for-bit, and
-  the plain outputs equal the oracle on seeded samples.  The test prints what the network gains (or
-  costs) against the plain walk: time ratio, rerun rate, and the walk's node visits.
+  capped search at m = 0.05, 0.1, 0.25, with the bare cap and with the calibrated slack.  Every learned
+  output equals the plain one bit-for-bit, and the plain outputs equal the oracle on seeded samples.
+  The test prints what the network gains (or costs) against the plain walk: time ratio, rerun rate,
+  and the walks' node visits.
 * Batched uneven workload (64 bumpy spheres, split levels 4..8, grids 32..192, 2^18..2^21 queries):
   one persistent launch; per item a seeded sample of 2,000 voxels and 2,000 points equals the oracle
   (magnitudes and records bit-exact, sign identical beyond 1e-3 voxel), plain and learned batches.
@@ -88,15 +89,44 @@ def test_learned_bound_k8_full_size():
     say(f"training: {t_train:.0f} ms, loss {loss0:.4g} -> {loss1:.4g}")
     assert loss1 < 0.1 * loss0
 
-    for m in (0.05, 0.1, 0.25):
-        d_proj.fill_(0)
-        t_lp, rp = timed(lambda: pr.find_projections_learned_device(net, m, dp.data_ptr(), nq, d_proj.data_ptr()))
-        d_sdf.fill_(0)
-        t_ls, rs = timed(lambda: fwn.sdf_grid_learned_device(net, m, dims, xf, opts, d_sdf.data_ptr()))
-        assert np.array_equal(d_proj.cpu().numpy(), plain_proj), m
-        assert np.array_equal(d_sdf.cpu().numpy().view(np.uint32), plain_sdf.view(np.uint32)), m
-        say(f"m={m}: projections {t_lp:.1f} ms ({t_pp / t_lp:.2f}x plain, {rp / nq:.3f} rerun), "
-            f"sdf {t_ls:.1f} ms ({t_ps / t_ls:.2f}x plain, {rs / nvox:.3f} rerun)")
+    # node visits of the walks (their own counters), plain against learned
+    def visits(fn):
+        buf = torch.zeros(M.STAT_LAUNCHES * M.STAT_STRIDE, dtype=torch.int64, device="cuda")
+        gm.set_stats(buf.data_ptr())
+        try:
+            fn()
+            torch.cuda.synchronize()
+        finally:
+            gm.set_stats(0)
+        c = buf.view(M.STAT_LAUNCHES, M.STAT_STRIDE).cpu().numpy()
+        return int(c[:, M.STAT_NAMES.index("keyed_steps")].sum() + c[:, M.STAT_NAMES.index("quad_steps")].sum())
+    v_plain = visits(lambda: (pr.find_projections_device(dp.data_ptr(), nq, d_proj.data_ptr()),
+                              fwn.sdf_grid_device(dims, xf, opts, d_sdf.data_ptr())))
+
+    # the bare cap (pred (1 + m))^2 and the calibrated one (slack: the 0.99 quantile of held-out
+    # under-predictions); the bare cap reruns many queries, which exercises the rerun passes
+    cp = torch.from_numpy(synth.projection_queries(v, f, 1 << 16, 99)).cuda()
+    crec = torch.empty((1 << 16) * 8, dtype=torch.int32, device="cuda")
+    pr.find_projections_device(cp.data_ptr(), 1 << 16, crec.data_ptr())
+    clab = crec.view(torch.float32).view(-1, 8)[:, 7].sqrt().contiguous()
+    slack = net.calibrate(cp, clab, 0.99)
+    assert slack > 0
+    for sl in (0.0, slack):
+        net.set_slack(sl)
+        for m in (0.05, 0.1, 0.25):
+            d_proj.fill_(0)
+            t_lp, rp = timed(lambda: pr.find_projections_learned_device(net, m, dp.data_ptr(), nq, d_proj.data_ptr()))
+            d_sdf.fill_(0)
+            t_ls, rs = timed(lambda: fwn.sdf_grid_learned_device(net, m, dims, xf, opts, d_sdf.data_ptr()))
+            assert np.array_equal(d_proj.cpu().numpy(), plain_proj), (sl, m)
+            assert np.array_equal(d_sdf.cpu().numpy().view(np.uint32), plain_sdf.view(np.uint32)), (sl, m)
+            if sl == 0.0 and m == 0.05:
+                assert rp > 0 and rs > 0      # the rerun passes ran
+            v_l = visits(lambda: (pr.find_projections_learned_device(net, m, dp.data_ptr(), nq, d_proj.data_ptr()),
+                                  fwn.sdf_grid_learned_device(net, m, dims, xf, opts, d_sdf.data_ptr())))
+            say(f"slack={sl:.4f} m={m}: projections {t_lp:.1f} ms ({t_pp / t_lp:.2f}x plain, {rp / nq:.4f} rerun), "
+                f"sdf {t_ls:.1f} ms ({t_ps / t_ls:.2f}x plain, {rs / nvox:.4f} rerun), cooperative node visits "
+                f"{v_l / v_plain:.3f} of plain (reruns walk per lane, uncounted)")
 
 
 @pytest.fixture(scope="module")
