@@ -47,6 +47,7 @@ QSEED = 7
 TRAIN_N, TRAIN_BATCH, TRAIN_LR, TRAIN_SEED = 1 << 20, 8192, 1e-3, 11
 MARGINS = (0.05, 0.1, 0.25)
 CAL_N, CAL_SEED, CAL_Q = 1 << 16, 99, 0.99
+BATCH_CAL_N = 1 << 14
 BATCH_TRAIN_N, BATCH_TRAIN_STEPS = 1 << 18, 500
 # FLOP per sample: forward 2*(99*128 + 3*128*128 + 128); backward 2 GEMMs per layer (dW, dH)
 MLP_FWD_FLOP = 2 * (99 * 128 + 3 * 128 * 128 + 128)
@@ -454,10 +455,16 @@ def batched_leg(args, M, torch, np, synth, timer):
             pr.find_projections_device(tp.data_ptr(), BATCH_TRAIN_N, trec.data_ptr())
             tlab = trec.view(torch.float32).view(-1, 8)[:, 7].sqrt().contiguous()
             net.train(tp, tlab, steps=BATCH_TRAIN_STEPS, batch=TRAIN_BATCH, lr=TRAIN_LR)
+            # slack from held-out labelled points, as for the single mesh
+            cp = torch.from_numpy(synth.projection_queries(v, f, BATCH_CAL_N, 5000 + i)).cuda()
+            crec = torch.empty(BATCH_CAL_N * 8, dtype=torch.int32, device="cuda")
+            pr.find_projections_device(cp.data_ptr(), BATCH_CAL_N, crec.data_ptr())
+            net.calibrate(cp, crec.view(torch.float32).view(-1, 8)[:, 7].sqrt().contiguous(), CAL_Q)
             nets.append(net)
         torch.cuda.synchronize()
         learned = {"train_s_all_meshes": time.time() - t0, "samples_per_mesh": BATCH_TRAIN_N,
-                   "steps_per_mesh": BATCH_TRAIN_STEPS, "margins": {}}
+                   "steps_per_mesh": BATCH_TRAIN_STEPS, "calibration_samples_per_mesh": BATCH_CAL_N,
+                   "calibration_quantile": CAL_Q, "margins": {}}
         for m in (0.1, 0.25):
             rr = [0]
 
@@ -546,7 +553,11 @@ def offset_leg(args, M, torch, np, synth, timer):
         tlab = trec.view(torch.float32).view(-1, 8)[:, 7].sqrt().contiguous()
         train_ms, _ = timer(lambda: net.train(tp, tlab, steps=args.train_steps, batch=TRAIN_BATCH, lr=TRAIN_LR,
                                               stream=sh))
-        learned = {"train_ms": train_ms, "train_steps": args.train_steps, "margins": {}}
+        cp = torch.from_numpy(synth.projection_queries(v, f, CAL_N, CAL_SEED)).cuda()
+        crec = torch.empty(CAL_N * 8, dtype=torch.int32, device="cuda")
+        proj.find_projections_device(cp.data_ptr(), CAL_N, crec.data_ptr(), stream=sh)
+        slack = net.calibrate(cp, crec.view(torch.float32).view(-1, 8)[:, 7].sqrt().contiguous(), CAL_Q, stream=sh)
+        learned = {"train_ms": train_ms, "train_steps": args.train_steps, "slack": slack, "margins": {}}
         for m in MARGINS:
             t, r = timer(lambda: fwn.sdf_grid_learned_device(net, m, dims, xf, opts, d_sdf.data_ptr(), stream=sh))
             same = bool(torch.equal(d_sdf.view(torch.int32), ref))

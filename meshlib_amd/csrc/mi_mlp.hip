@@ -28,8 +28,6 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int LDP = kHidden + 8;   // padded LDS row (BF16 elements) for 128-wide rows
-constexpr int LDQ = kTileRows + 8; // padded LDS row for 64-wide (transposed) rows
-constexpr int HP = kHidden + 4;    // padded FP32 row
 
 __device__ __forceinline__ uint16_t f2bf(float x)
 {
@@ -71,183 +69,156 @@ __device__ __forceinline__ void normalised(const Params& p, const float* pt, flo
     x[2] = (pt[2] - p.center[2]) * p.invScale;
 }
 
-// one 64-row tile through the four hidden layers; Act holds the layer input on entry and the
-// last hidden activation on exit
-template <bool TRAIN>
-__device__ __forceinline__ void hiddenLayers(const Params& p, uint16_t* Ws, uint16_t* Act, int64_t row0, int64_t n,
-                                             float* Zs, uint16_t* Hs)
+// stages layer l's BF16 weights (128 x 128, rows of `src`) into LDS rows padded to LDP
+__device__ __forceinline__ void stageWeights(const uint16_t* __restrict__ src, uint16_t* Ws)
 {
+    const int tid = threadIdx.x;
+    for (int it = 0; it < 8; ++it) {          // 128 x 128 BF16 = 2048 chunks of 16 B
+        const int c = it * 256 + tid, row = c >> 4, c8 = (c & 15) * 8;
+        *reinterpret_cast<uint4*>(&Ws[row * LDP + c8]) = *reinterpret_cast<const uint4*>(&src[row * kHidden + c8]);
+    }
+}
+
+// 8 BF16 of column `col` in rows r .. r + 7 of a row-major LDS array (a K-major MFMA fragment)
+__device__ __forceinline__ bf16x8 colFrag(const uint16_t* A, int r, int col)
+{
+    uint16_t h[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) h[e] = A[(r + e) * LDP + col];
+    const uint4 v = make_uint4((uint32_t)h[0] | ((uint32_t)h[1] << 16), (uint32_t)h[2] | ((uint32_t)h[3] << 16),
+                               (uint32_t)h[4] | ((uint32_t)h[5] << 16), (uint32_t)h[6] | ((uint32_t)h[7] << 16));
+    return __builtin_bit_cast(bf16x8, v);
+}
+
+// One training step's forward and backward pass over a 64-row tile, in one launch.  loss =
+// 1/(2B) sum (pred/scale - label/scale)^2 over the batch; writes the tile's partial gradient.
+// The BF16 activations H_{-1} (features) .. H_3 stay in LDS; each layer's pre-activations Z are computed
+// again in the backward pass from the same MFMA inputs in the same order (so the same values) instead
+// of being stored; dH and dZ stay in registers in the accumulator layout (wave w: rows 16w + 4(l/16) + j,
+// column 16 nt + l%16), dZ also in LDS as BF16 for the two products dW = dZ^T H and dH = dZ W.
+// Rounding points as the oracle: features, weights, activations and dZ (for the products) in BF16,
+// FP32 accumulation, the bias gradients from FP32 dZ.
+__global__ __launch_bounds__(256) void kTrainTile(Params p, const float* __restrict__ pts,
+                                                  const float* __restrict__ labels, int64_t n, float invB,
+                                                  float* __restrict__ partial)
+{
+    __shared__ __attribute__((aligned(16))) uint16_t Act[kLayers + 1][kTileRows * LDP];
+    __shared__ __attribute__((aligned(16))) uint16_t Ws[kHidden * LDP];
+    __shared__ __attribute__((aligned(16))) uint16_t dZr[kTileRows * LDP];
+    __shared__ float duS[kTileRows];
+    __shared__ float red[4][kHidden];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int lr = lane & 15, lk = lane >> 4;
     const int r0 = 16 * w;
-    for (int l = 0; l < kLayers; ++l) {
-        __syncthreads();
-        const uint16_t* W = p.Wb + (size_t)l * kHidden * kHidden;
-        for (int it = 0; it < 8; ++it) {          // 128 x 128 BF16 = 2048 chunks of 16 B
-            const int c = it * 256 + tid, row = c >> 4, c8 = (c & 15) * 8;
-            *reinterpret_cast<uint4*>(&Ws[row * LDP + c8]) = *reinterpret_cast<const uint4*>(&W[row * kHidden + c8]);
-        }
-        __syncthreads();
-        bf16x8 a[4];
-#pragma unroll
-        for (int kc = 0; kc < 4; ++kc) a[kc] = frag(&Act[(r0 + lr) * LDP + kc * 32 + lk * 8]);
-        f32x4 acc[8];
-#pragma unroll
-        for (int nt = 0; nt < 8; ++nt) {
-            acc[nt] = f32x4{ 0.f, 0.f, 0.f, 0.f };
-#pragma unroll
-            for (int kc = 0; kc < 4; ++kc) acc[nt] = mfma(a[kc], frag(&Ws[(nt * 16 + lr) * LDP + kc * 32 + lk * 8]), acc[nt]);
-        }
-        const float* bias = p.theta + kOffB + l * kHidden;
-#pragma unroll
-        for (int nt = 0; nt < 8; ++nt) {
-            const int col = nt * 16 + lr;
-            const float bc = bias[col];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int row = r0 + 4 * lk + j;
-                const float z = acc[nt][j] + bc;
-                const uint16_t hb = f2bf(siluf(z));
-                Act[row * LDP + col] = hb;
-                if (TRAIN) {
-                    const int64_t gi = row0 + row;
-                    if (gi < n) {
-                        Zs[((size_t)l * n + gi) * kHidden + col] = z;
-                        Hs[((size_t)l * n + gi) * kHidden + col] = hb;
-                    }
-                }
-            }
-        }
-    }
-    __syncthreads();
-}
-
-// training forward: keeps each layer's pre-activations Z and BF16 activations H, and the output's U
-__global__ __launch_bounds__(256) void kForwardTrain(Params p, const float* __restrict__ pts, int64_t n,
-                                                     float* __restrict__ Zs, uint16_t* __restrict__ Hs,
-                                                     float* __restrict__ U)
-{
-    __shared__ __attribute__((aligned(16))) uint16_t Ws[kHidden * LDP];
-    __shared__ __attribute__((aligned(16))) uint16_t Act[kTileRows * LDP];
-    const int tid = threadIdx.x;
     const int64_t row0 = (int64_t)blockIdx.x * kTileRows;
-    {
-        const int r = tid >> 2, part = tid & 3;
+    float* part = partial + (size_t)blockIdx.x * kParamsPadded;
+
+    {   // features (rows past n: the features of the origin; their loss weight is 0)
+        const int r = tid >> 2, q4 = tid & 3;
         const int64_t gi = row0 + r;
         float x[3] = { 0.f, 0.f, 0.f };
         if (gi < n) normalised(p, pts + 3 * gi, x);
         for (int j = 0; j < 32; ++j) {
-            const int fi = 32 * part + j;
-            Act[r * LDP + fi] = f2bf(feature(fi, x, p.piFreq));
+            const int fi = 32 * q4 + j;
+            Act[0][r * LDP + fi] = f2bf(feature(fi, x, p.piFreq));
         }
     }
-    hiddenLayers<true>(p, Ws, Act, row0, n, Zs, Hs);
-    // output layer: u = sum_c h4[c] * w5[c] + b5 (four partial sums of 32, combined pairwise)
-    const int r = tid >> 2, part = tid & 3;
-    float s = 0.f;
-    for (int j = 0; j < 32; ++j) {
-        const int c = 32 * part + j;
-        s += bf2f(Act[r * LDP + c]) * bf2f(p.w5b[c]);
-    }
-    s += __shfl_xor(s, 1);
-    s += __shfl_xor(s, 2);
-    const int64_t gi = row0 + r;
-    if (part != 0 || gi >= n) return;
-    U[gi] = s + p.theta[kOffB5];
-}
-
-// loss = 1/(2B) * sum (pred/scale - label/scale)^2 over the batch; per-tile partial gradients
-__global__ __launch_bounds__(256) void kBackward(Params p, const float* __restrict__ pts,
-                                                 const float* __restrict__ labels, int64_t n, float invB,
-                                                 const float* __restrict__ Zs, const uint16_t* __restrict__ Hs,
-                                                 const float* __restrict__ U, float* __restrict__ partial)
-{
-    __shared__ __attribute__((aligned(16))) uint16_t WT[kHidden * LDP];
-    __shared__ __attribute__((aligned(16))) uint16_t dZr[kTileRows * LDP];
-    __shared__ __attribute__((aligned(16))) uint16_t dZT[kHidden * LDQ];
-    __shared__ __attribute__((aligned(16))) uint16_t HT[kHidden * LDQ];
-    __shared__ float dH[kTileRows * HP];
-    __shared__ float duS[kTileRows];
-    __shared__ float red[2][kHidden];
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int lr = lane & 15, lk = lane >> 4;
-    const int64_t row0 = (int64_t)blockIdx.x * kTileRows;
-    float* part = partial + (size_t)blockIdx.x * kParamsPadded;
-
-    if (tid < kTileRows) {
-        const int64_t gi = row0 + tid;
-        float du = 0.f;
-        if (gi < n) {
-            const float u = U[gi];
-            const float diff = softplusf(u) - labels[gi] * p.invScale;
-            du = diff * invB * sigmoidf(u);
+    // forward: H_l = BF16(SiLU(H_{l-1} W_l^T + b_l))
+    for (int l = 0; l < kLayers; ++l) {
+        __syncthreads();
+        stageWeights(p.Wb + (size_t)l * kHidden * kHidden, Ws);
+        __syncthreads();
+        bf16x8 a[4];
+#pragma unroll
+        for (int kc = 0; kc < 4; ++kc) a[kc] = frag(&Act[l][(r0 + lr) * LDP + kc * 32 + lk * 8]);
+        const float* bias = p.theta + kOffB + l * kHidden;
+#pragma unroll
+        for (int nt = 0; nt < 8; ++nt) {
+            f32x4 acc = f32x4{ 0.f, 0.f, 0.f, 0.f };
+#pragma unroll
+            for (int kc = 0; kc < 4; ++kc) acc = mfma(a[kc], frag(&Ws[(nt * 16 + lr) * LDP + kc * 32 + lk * 8]), acc);
+            const int col = nt * 16 + lr;
+            const float bc = bias[col];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) Act[l + 1][(r0 + 4 * lk + j) * LDP + col] = f2bf(siluf(acc[j] + bc));
         }
-        duS[tid] = du;
     }
     __syncthreads();
-    // output layer: dw5[c] = sum_r du[r] h4[r][c], db5 = sum_r du[r], dH4[r][c] = du[r] w5[c]
-    {
+    {   // output layer u = sum_c h3[c] w5[c] + b5 and du = dloss/du (four partial sums of 32 per row)
+        const int r = tid >> 2, q4 = tid & 3;
+        float sum = 0.f;
+        for (int j = 0; j < 32; ++j) {
+            const int c = 32 * q4 + j;
+            sum += bf2f(Act[kLayers][r * LDP + c]) * bf2f(p.w5b[c]);
+        }
+        sum += __shfl_xor(sum, 1);
+        sum += __shfl_xor(sum, 2);
+        if (q4 == 0) {
+            const int64_t gi = row0 + r;
+            float du = 0.f;
+            if (gi < n) {
+                const float u = sum + p.theta[kOffB5];
+                du = (softplusf(u) - labels[gi] * p.invScale) * invB * sigmoidf(u);
+            }
+            duS[r] = du;
+        }
+    }
+    __syncthreads();
+    {   // dw5[c] = sum_r du[r] h3[r][c], db5 = sum_r du[r]
         const int c = tid & 127, half = tid >> 7;
-        float s = 0.f;
+        float sum = 0.f;
         for (int rr = 0; rr < 32; ++rr) {
             const int r = half * 32 + rr;
-            const int64_t gi = row0 + r;
-            if (gi < n) s += duS[r] * bf2f(Hs[((size_t)(kLayers - 1) * n + gi) * kHidden + c]);
+            sum += duS[r] * bf2f(Act[kLayers][r * LDP + c]);
         }
-        red[half][c] = s;
-    }
-    for (int e = tid; e < kTileRows * kHidden; e += 256) {
-        const int r = e >> 7, c = e & 127;
-        dH[r * HP + c] = duS[r] * bf2f(p.w5b[c]);
+        red[half][c] = sum;
     }
     __syncthreads();
     if (tid < kHidden) part[kOffW5 + tid] = red[0][tid] + red[1][tid];
     if (tid == 0) {
-        float s = 0.f;
-        for (int r = 0; r < kTileRows; ++r) s += duS[r];
-        part[kOffB5] = s;
+        float sum = 0.f;
+        for (int r = 0; r < kTileRows; ++r) sum += duS[r];
+        part[kOffB5] = sum;
     }
+    // dH_3 = du w5^T, in registers
+    float dH[8][4];
+#pragma unroll
+    for (int nt = 0; nt < 8; ++nt)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) dH[nt][j] = duS[r0 + 4 * lk + j] * bf2f(p.w5b[nt * 16 + lr]);
     for (int l = kLayers - 1; l >= 0; --l) {
-        // dZ = dH * silu'(Z), kept as FP32 in dH (for the bias gradient) and as BF16 in both layouts
-        for (int e = tid; e < kTileRows * kHidden; e += 256) {
-            const int r = e >> 7, c = e & 127;
-            const int64_t gi = row0 + r;
-            float dz = 0.f;
-            if (gi < n) {
-                const float z = Zs[((size_t)l * n + gi) * kHidden + c];
-                const float s = sigmoidf(z);
-                dz = dH[r * HP + c] * (s * (1.f + z * (1.f - s)));
+        __syncthreads();      // the previous layer's readers of Ws, dZr and red are done
+        stageWeights(p.Wb + (size_t)l * kHidden * kHidden, Ws);
+        __syncthreads();
+        // Z_l again, then dZ = dH * SiLU'(Z): FP32 for the bias gradient, BF16 in dZr for the products
+        bf16x8 a[4];
+#pragma unroll
+        for (int kc = 0; kc < 4; ++kc) a[kc] = frag(&Act[l][(r0 + lr) * LDP + kc * 32 + lk * 8]);
+        const float* bias = p.theta + kOffB + l * kHidden;
+#pragma unroll
+        for (int nt = 0; nt < 8; ++nt) {
+            f32x4 acc = f32x4{ 0.f, 0.f, 0.f, 0.f };
+#pragma unroll
+            for (int kc = 0; kc < 4; ++kc) acc = mfma(a[kc], frag(&Ws[(nt * 16 + lr) * LDP + kc * 32 + lk * 8]), acc);
+            const int col = nt * 16 + lr;
+            const float bc = bias[col];
+            float colSum = 0.f;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const float z = acc[j] + bc;
+                const float sg = sigmoidf(z);
+                const float dz = dH[nt][j] * (sg * (1.f + z * (1.f - sg)));
+                dZr[(r0 + 4 * lk + j) * LDP + col] = f2bf(dz);
+                colSum += dz;
             }
-            dH[r * HP + c] = dz;
-            const uint16_t b = f2bf(dz);
-            dZr[r * LDP + c] = b;
-            dZT[c * LDQ + r] = b;
-        }
-        // the layer input, transposed: H_{l-1} or (l == 0) the features
-        if (l > 0) {
-            for (int e = tid; e < kTileRows * kHidden; e += 256) {
-                const int r = e >> 7, c = e & 127;
-                const int64_t gi = row0 + r;
-                HT[c * LDQ + r] = gi < n ? Hs[((size_t)(l - 1) * n + gi) * kHidden + c] : (uint16_t)0;
-            }
-        } else {
-            const int r = tid >> 2, q4 = tid & 3;
-            const int64_t gi = row0 + r;
-            float x[3] = { 0.f, 0.f, 0.f };
-            if (gi < n) normalised(p, pts + 3 * gi, x);
-            for (int j = 0; j < 32; ++j) {
-                const int fi = 32 * q4 + j;
-                HT[fi * LDQ + r] = gi < n ? f2bf(feature(fi, x, p.piFreq)) : (uint16_t)0;
-            }
+            // column sums over the wave's 16 rows (lanes l, l^16, l^32, l^48 hold the same column)
+            colSum += __shfl_xor(colSum, 16);
+            colSum += __shfl_xor(colSum, 32);
+            if (lk == 0) red[w][col] = colSum;
         }
         __syncthreads();
-        if (tid < kHidden) {
-            float s = 0.f;
-            for (int r = 0; r < kTileRows; ++r) s += dH[r * HP + tid];
-            part[kOffB + l * kHidden + tid] = s;
-        }
-        // dW_l (128 out x 128 in) = dZ^T (out x rows) . H (rows x in); wave w: out rows [32w, 32w+32)
-        {
+        if (tid < kHidden) part[kOffB + l * kHidden + tid] = ((red[0][tid] + red[1][tid]) + red[2][tid]) + red[3][tid];
+        {   // dW_l (128 out x 128 in) = dZ^T (out x rows) . H_{l-1} (rows x in); wave w: out rows [32w, 32w+32)
             f32x4 acc[2][8];
 #pragma unroll
             for (int mt = 0; mt < 2; ++mt)
@@ -255,11 +226,11 @@ __global__ __launch_bounds__(256) void kBackward(Params p, const float* __restri
                 for (int nt = 0; nt < 8; ++nt) acc[mt][nt] = f32x4{ 0.f, 0.f, 0.f, 0.f };
 #pragma unroll
             for (int kc = 0; kc < 2; ++kc) {
-                bf16x8 a0 = frag(&dZT[(32 * w + lr) * LDQ + kc * 32 + lk * 8]);
-                bf16x8 a1 = frag(&dZT[(32 * w + 16 + lr) * LDQ + kc * 32 + lk * 8]);
+                const bf16x8 a0 = colFrag(dZr, 32 * kc + 8 * lk, 32 * w + lr);
+                const bf16x8 a1 = colFrag(dZr, 32 * kc + 8 * lk, 32 * w + 16 + lr);
 #pragma unroll
                 for (int nt = 0; nt < 8; ++nt) {
-                    const bf16x8 b = frag(&HT[(16 * nt + lr) * LDQ + kc * 32 + lk * 8]);
+                    const bf16x8 b = colFrag(Act[l], 32 * kc + 8 * lk, 16 * nt + lr);
                     acc[0][nt] = mfma(a0, b, acc[0][nt]);
                     acc[1][nt] = mfma(a1, b, acc[1][nt]);
                 }
@@ -274,32 +245,21 @@ __global__ __launch_bounds__(256) void kBackward(Params p, const float* __restri
                         dW[(32 * w + 16 * mt + 4 * lk + j) * kHidden + 16 * nt + lr] = acc[mt][nt][j];
         }
         if (l == 0) break;
-        // dH_{l-1} (rows x in) = dZ (rows x out) . W_l (out x in); B(k = out, n = in) = W^T[in][out]
-        const uint16_t* Wt = p.WbT + (size_t)l * kHidden * kHidden;
-        for (int it = 0; it < 8; ++it) {
-            const int c = it * 256 + tid, row = c >> 4, c8 = (c & 15) * 8;
-            *reinterpret_cast<uint4*>(&WT[row * LDP + c8]) = *reinterpret_cast<const uint4*>(&Wt[row * kHidden + c8]);
-        }
+        // dH_{l-1} (rows x in) = dZ (rows x out) . W_l (out x in); B(k = out, n = in) = W^T[in][out].  Every
+        // wave finished reading Ws (W_l) before the barrier above.
+        stageWeights(p.WbT + (size_t)l * kHidden * kHidden, Ws);
         __syncthreads();
-        {
-            f32x4 acc[8];
-            bf16x8 a[4];
+        bf16x8 d[4];
 #pragma unroll
-            for (int kc = 0; kc < 4; ++kc) a[kc] = frag(&dZr[(16 * w + lr) * LDP + kc * 32 + lk * 8]);
+        for (int kc = 0; kc < 4; ++kc) d[kc] = frag(&dZr[(r0 + lr) * LDP + kc * 32 + lk * 8]);
 #pragma unroll
-            for (int nt = 0; nt < 8; ++nt) {
-                acc[nt] = f32x4{ 0.f, 0.f, 0.f, 0.f };
+        for (int nt = 0; nt < 8; ++nt) {
+            f32x4 acc = f32x4{ 0.f, 0.f, 0.f, 0.f };
 #pragma unroll
-                for (int kc = 0; kc < 4; ++kc)
-                    acc[nt] = mfma(a[kc], frag(&WT[(16 * nt + lr) * LDP + kc * 32 + lk * 8]), acc[nt]);
-            }
-            __syncthreads();   // every wave is done reading dH (bias sums) before it is overwritten
+            for (int kc = 0; kc < 4; ++kc) acc = mfma(d[kc], frag(&Ws[(nt * 16 + lr) * LDP + kc * 32 + lk * 8]), acc);
 #pragma unroll
-            for (int nt = 0; nt < 8; ++nt)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) dH[(16 * w + 4 * lk + j) * HP + 16 * nt + lr] = acc[nt][j];
+            for (int j = 0; j < 4; ++j) dH[nt][j] = acc[j];
         }
-        __syncthreads();
     }
 }
 
@@ -532,12 +492,23 @@ __device__ __forceinline__ void refresh(const Params& p, int j, float v)
     }
 }
 
-__global__ void kAdam(Params p, const float* __restrict__ grad, float* __restrict__ mom, float* __restrict__ vel,
-                      float lr, float b1, float b2, float eps, float c1, float c2)
+// one Adam step straight from the backward pass's per-tile partial gradients (their sum in four
+// interleaved accumulators, then the update): no separate reduction launch
+__global__ void kAdamPartials(Params p, const float* __restrict__ partial, int nBlocks, float* __restrict__ mom,
+                              float* __restrict__ vel, float lr, float b1, float b2, float eps, float c1, float c2)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= kParams) return;
-    const float g = grad[j];
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+    int b = 0;
+    for (; b + 4 <= nBlocks; b += 4) {
+        s0 += partial[(size_t)b * kParamsPadded + j];
+        s1 += partial[(size_t)(b + 1) * kParamsPadded + j];
+        s2 += partial[(size_t)(b + 2) * kParamsPadded + j];
+        s3 += partial[(size_t)(b + 3) * kParamsPadded + j];
+    }
+    for (; b < nBlocks; ++b) s0 += partial[(size_t)b * kParamsPadded + j];
+    const float g = (s0 + s1) + (s2 + s3);
     const float m = b1 * mom[j] + (1.f - b1) * g;
     const float v = b2 * vel[j] + (1.f - b2) * g * g;
     mom[j] = m;
@@ -616,23 +587,26 @@ hipError_t gradient(const Params& p, const float* pts, const float* labels, int6
 {
     if (n <= 0) return hipSuccess;
     const int blocks = (int)((n + kTileRows - 1) / kTileRows);
-    hipLaunchKernelGGL(kForwardTrain, dim3(blocks), dim3(256), 0, s, p, pts, n, tb.Z, tb.H, tb.U);
+    hipLaunchKernelGGL(kTrainTile, dim3(blocks), dim3(256), 0, s, p, pts, labels, n, 1.f / (float)n, tb.partial);
     hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(kBackward, dim3(blocks), dim3(256), 0, s, p, pts, labels, n, 1.f / (float)n, tb.Z, tb.H,
-                       tb.U, tb.partial);
-    e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(kReduce, dim3((kParamsPadded + 255) / 256), dim3(256), 0, s, tb.partial, blocks, grad);
     return hipGetLastError();
 }
 
-hipError_t adamStep(const Params& p, const float* grad, float* mom, float* vel, int step, float lr, hipStream_t s)
+hipError_t trainStep(const Params& p, const float* pts, const float* labels, int64_t n, const TrainBuffers& tb,
+                     float* mom, float* vel, int step, float lr, hipStream_t s)
 {
+    if (n <= 0) return hipSuccess;
+    const int blocks = (int)((n + kTileRows - 1) / kTileRows);
+    hipLaunchKernelGGL(kTrainTile, dim3(blocks), dim3(256), 0, s, p, pts, labels, n, 1.f / (float)n, tb.partial);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
     const float c1 = (float)(1.0 / (1.0 - std::pow(0.9, step)));
     const float c2 = (float)(1.0 / (1.0 - std::pow(0.999, step)));
-    hipLaunchKernelGGL(kAdam, dim3((kParams + 255) / 256), dim3(256), 0, s, p, grad, mom, vel, lr, b1, b2, eps, c1, c2);
+    hipLaunchKernelGGL(kAdamPartials, dim3((kParams + 255) / 256), dim3(256), 0, s, p, tb.partial, blocks, mom, vel,
+                       lr, b1, b2, eps, c1, c2);
     return hipGetLastError();
 }
 

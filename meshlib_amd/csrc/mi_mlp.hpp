@@ -38,9 +38,6 @@ struct Params {
 };
 
 struct TrainBuffers {
-    float* Z;          // [kLayers][n][kHidden] pre-activations
-    uint16_t* H;       // [kLayers][n][kHidden] BF16 activations
-    float* U;          // [n] output pre-softplus
     float* partial;    // [ceil(n / kTileRows)][kParamsPadded] per-tile gradients
 };
 
@@ -58,7 +55,9 @@ int64_t gridCapBricks(const int32_t dims[3], int32_t zBegin, int32_t zEnd);
 // gradient of 1/(2n) sum (pred/scale - label/scale)^2 over the n points into grad[kParamsPadded]
 hipError_t gradient(const Params& p, const float* pts, const float* labels, int64_t n, const TrainBuffers& tb,
                     float* grad, hipStream_t s);
-hipError_t adamStep(const Params& p, const float* grad, float* mom, float* vel, int step, float lr, hipStream_t s);
+// one training step (forward, backward, Adam on the summed per-tile gradients) over n points
+hipError_t trainStep(const Params& p, const float* pts, const float* labels, int64_t n, const TrainBuffers& tb,
+                     float* mom, float* vel, int step, float lr, hipStream_t s);
 hipError_t refreshCopies(const Params& p, hipStream_t s);
 
 } // namespace mlp

@@ -63,7 +63,7 @@ struct Rng {
 struct mi_mlp {
     int device = 0;
     Buf theta, mom, vel, grad, Wb, WbT, w5b;
-    Buf trainZ, trainH, trainU, trainPartial;
+    Buf trainPartial;
     Buf work;      // predictions / caps / grid points of the learned-bound forms
     int64_t trainRows = 0;
     int step = 0;
@@ -171,9 +171,6 @@ static int ensureTrain(mi_mlp* m, int64_t rows)
     using namespace mi::mlp;
     if (rows <= m->trainRows) return MI_OK;
     const int64_t tiles = (rows + kTileRows - 1) / kTileRows;
-    MLP_HIP(m->trainZ.ensure((size_t)kLayers * rows * kHidden * 4));
-    MLP_HIP(m->trainH.ensure((size_t)kLayers * rows * kHidden * 2));
-    MLP_HIP(m->trainU.ensure((size_t)rows * 4));
     MLP_HIP(m->trainPartial.ensure((size_t)tiles * kParamsPadded * 4));
     m->trainRows = rows;
     return MI_OK;
@@ -181,7 +178,7 @@ static int ensureTrain(mi_mlp* m, int64_t rows)
 
 static mi::mlp::TrainBuffers trainBuffers(mi_mlp* m)
 {
-    return { m->trainZ.as<float>(), m->trainH.as<uint16_t>(), m->trainU.as<float>(), m->trainPartial.as<float>() };
+    return { m->trainPartial.as<float>() };
 }
 
 int mi_mlp_gradient_device(mi_mlp* m, const float* d_pts, const float* d_labels, int64_t n, float* d_grad,
@@ -208,10 +205,8 @@ int mi_mlp_train_device(mi_mlp* m, const float* d_pts, const float* d_labels, in
     const int64_t nBatches = n / batch;
     for (int32_t it = 0; it < steps; ++it) {
         const int64_t off = (int64_t)(it % nBatches) * batch;
-        MLP_HIP(mi::mlp::gradient(m->params, d_pts + 3 * off, d_labels + off, batch, trainBuffers(m),
-                                  m->grad.as<float>(), s));
-        MLP_HIP(mi::mlp::adamStep(m->params, m->grad.as<float>(), m->mom.as<float>(), m->vel.as<float>(), ++m->step,
-                                  lr, s));
+        MLP_HIP(mi::mlp::trainStep(m->params, d_pts + 3 * off, d_labels + off, batch, trainBuffers(m),
+                                   m->mom.as<float>(), m->vel.as<float>(), ++m->step, lr, s));
     }
     return MI_OK;
 }
