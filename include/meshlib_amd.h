@@ -323,10 +323,10 @@ int  mi_batch_set_config(mi_batch* b, int32_t blocksPerCU, int32_t xcdSplit);
 int  mi_batch_get_config(mi_batch* b, int32_t* blocksPerCU, int32_t* xcdSplit);
 /* the batch with per-query caps (squared) on the search radius: d_capSdf[i] float[voxels of item i]
  * and d_capProj[i] float[numPts of item i] (device; either array or entry may be NULL = no cap).
- * Pass 1 is one persistent launch with the caps; pass 2 one persistent launch, uncapped, over the
- * queries that found nothing below a cap smaller than the plain limit (*reruns = their count).
- * Any cap at or above a query's true distance gives the plain result, so outputs equal
- * mi_batch_run_device's.  Synchronises the stream between the passes. */
+ * One persistent launch with the caps; a wave whose queries found nothing below a cap smaller than
+ * the plain limit walks once more for them, uncapped (*reruns = their count).  Any cap at or above a
+ * query's true distance gives the plain result, so outputs equal mi_batch_run_device's.
+ * Synchronises the stream (to read the count). */
 int  mi_batch_run_capped_device(mi_batch* b, const mi_batch_item* items, int32_t count, const float* const* d_capSdf,
                                 const float* const* d_capProj, const mi_dist_options* opts, float upDistLimitSq,
                                 void* stream, int64_t* reruns);

@@ -500,12 +500,16 @@ class DistanceMLP:
         assert th.size == self.num_params
         _check(lib().mi_mlp_set_params(self._h, _p(th)))
 
+    def predict_device(self, d_pts, d_out, stream: int = 0):
+        """Predicted distances of the device points (float32 [n, 3]) into d_out (float32 [n], device),
+        asynchronous on `stream`."""
+        _check(lib().mi_mlp_predict_device(self._h, C.c_void_p(d_pts.data_ptr()), int(d_pts.shape[0]),
+                                           C.c_void_p(d_out.data_ptr()), C.c_void_p(stream or None)))
+
     def predict(self, d_pts, stream: int = 0):
         import torch
-        n = d_pts.shape[0]
-        out = torch.empty(n, dtype=torch.float32, device=d_pts.device)
-        _check(lib().mi_mlp_predict_device(self._h, C.c_void_p(d_pts.data_ptr()), n, C.c_void_p(out.data_ptr()),
-                                           C.c_void_p(stream or None)))
+        out = torch.empty(d_pts.shape[0], dtype=torch.float32, device=d_pts.device)
+        self.predict_device(d_pts, out, stream)
         torch.cuda.synchronize()
         return out.cpu().numpy()
 

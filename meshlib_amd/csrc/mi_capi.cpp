@@ -1158,7 +1158,7 @@ struct mi_batch {
     DevBuf items;        // BatchItem[]
     DevBuf tickets;      // 8 x u64
     DevBuf sortWs;       // per-item Morton sort workspaces
-    DevBuf rerunWs;      // capped form: per-item rerun counters and lists
+    DevBuf rerunWs;      // capped form: per-item counters of failed caps
     std::vector<mi::BatchItem> host;     // launch order (see prepareBatch)
     std::vector<int32_t> slotOf;          // caller's item index -> position in `host`
     std::mutex mu;
@@ -1244,13 +1244,13 @@ int prepareBatch(mi_batch* b, const mi_batch_item* items, int32_t count, hipStre
 }
 
 int launchPrepared(mi_batch* b, int32_t count, int64_t total, int stackD, const mi_dist_options* opts, float up,
-                   int blocksPerCU, int xcdSplit, hipStream_t s)
+                   int blocksPerCU, int xcdSplit, hipStream_t s, bool capped = false)
 {
     mi::DistOpts o{ opts->minDistSq, opts->maxDistSq, opts->nullOutsideMinMax, opts->windingNumberThreshold,
                     opts->windingNumberBeta };
     MI_HIP(mi::fillU32(b->tickets.as<uint32_t>(), 16, 0u, s));
     MI_HIP(mi::launchBatch(b->items.as<mi::BatchItem>(), count, total, o, up,
-                           b->tickets.as<unsigned long long>(), xcdSplit, b->numCUs * blocksPerCU, stackD, s));
+                           b->tickets.as<unsigned long long>(), xcdSplit, b->numCUs * blocksPerCU, stackD, capped, s));
     return MI_OK;
 }
 
@@ -1326,57 +1326,35 @@ int mi_batch_run_capped_device(mi_batch* b, const mi_batch_item* items, int32_t 
     int64_t total = 0;
     int stackD = 8;
     if (int rc = prepareBatch(b, items, count, s, &total, &stackD, false)) return rc;
-    // rerun storage: 2 counters per item, then per item a voxel list and a point list
+    // two counters per item: the voxels and points whose cap failed (each searched again, uncapped, by its
+    // own wave inside the launch)
     const size_t cntBytes = align256((size_t)count * 8);
-    size_t need = cntBytes;
+    if (b->rerunWs.bytes < cntBytes) MI_HIP(b->rerunWs.alloc(cntBytes));
+    uint32_t* counts = b->rerunWs.as<uint32_t>();
     for (int32_t i = 0; i < count; ++i) {
-        const mi_batch_item& it = items[i];
-        need += align256((size_t)it.dims[0] * it.dims[1] * it.dims[2] * 4) + align256((size_t)it.numPts * 4);
-    }
-    if (b->rerunWs.bytes < need) MI_HIP(b->rerunWs.alloc(need));
-    char* base = b->rerunWs.as<char>();
-    uint32_t* counts = reinterpret_cast<uint32_t*>(base);
-    size_t off = cntBytes;
-    for (int32_t i = 0; i < count; ++i) {
-        const mi_batch_item& it = items[i];
         const int32_t k = b->slotOf[(size_t)i];
         mi::BatchItem& d = b->host[(size_t)k];
-        const size_t nvox = (size_t)it.dims[0] * it.dims[1] * it.dims[2];
         d.capSdf = d_capSdf ? d_capSdf[i] : nullptr;
         d.capProj = d_capProj ? d_capProj[i] : nullptr;
-        d.rerunVox = reinterpret_cast<uint32_t*>(base + off);
-        off += align256(nvox * 4);
-        d.rerunPts = reinterpret_cast<uint32_t*>(base + off);
-        off += align256((size_t)it.numPts * 4);
         d.rerunCount = counts + 2 * k;
     }
     MI_HIP(mi::fillU32(counts, 2 * count, 0u, s));
     if (int rc = uploadItems(b, count, s)) return rc;
-    if (int rc = launchPrepared(b, count, total, stackD, opts, upDistLimitSq, b->blocksPerCU, b->xcdSplit, s)) return rc;
-    // second pass: one persistent launch over the queries whose cap was too small, uncapped
+    if (int rc = launchPrepared(b, count, total, stackD, opts, upDistLimitSq, b->blocksPerCU, b->xcdSplit, s, true))
+        return rc;
     std::vector<uint32_t> hc((size_t)count * 2);
     MI_HIP(hipMemcpyAsync(hc.data(), counts, hc.size() * 4, hipMemcpyDeviceToHost, s));
     MI_HIP(hipStreamSynchronize(s));
-    int64_t unit = 0, again = 0;
-    for (int32_t k = 0; k < count; ++k) {     // launch order, as the counters
-        mi::BatchItem& d = b->host[(size_t)k];
-        const uint32_t nv = hc[2 * (size_t)k], np = hc[2 * (size_t)k + 1];
-        again += (int64_t)nv + np;
-        d.voxList = d.rerunVox;
-        d.nVoxList = nv;
-        d.gridUnits = (nv + 63) / 64;
-        d.perm = d.rerunPts;
-        d.npts = np;
-        d.projUnits = (np + 63) / 64;
-        d.capSdf = d.capProj = nullptr;
-        d.rerunVox = d.rerunPts = d.rerunCount = nullptr;
-        d.unitBegin = unit;
-        unit += d.gridUnits + d.projUnits;
-    }
+    int64_t again = 0;
+    for (uint32_t c : hc) again += c;
     if (reruns) *reruns = again;
-    if (unit == 0) return MI_OK;
-    if (int rc = uploadItems(b, count, s)) return rc;
-    return launchPrepared(b, count, unit, stackD, opts, upDistLimitSq, b->blocksPerCU, b->xcdSplit, s);
+    // the items' descriptors go back to the plain form for later calls
+    for (int32_t k = 0; k < count; ++k) {
+        mi::BatchItem& d = b->host[(size_t)k];
+        d.capSdf = d.capProj = nullptr;
+        d.rerunCount = nullptr;
+    }
+    return MI_OK;
 }
 
 int mi_batch_autotune(mi_batch* b, const mi_batch_item* items, int32_t count, const mi_dist_options* opts,

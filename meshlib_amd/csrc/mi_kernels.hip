@@ -332,7 +332,10 @@ __global__ __launch_bounds__(kBlock) void kSelfIntersectionsCoop(MeshDev m, int 
 // range is cut into 8 contiguous parts, one per XCD (blocks are dealt to XCDs round-robin), and a
 // wave whose part is drained moves on to the next part: spatially adjacent units share an L2,
 // and the tail is balanced.  Every wave exits once it has seen every part drained.
-template <int D>
+// CAPPED: the items' caps apply; a wave whose lanes found nothing below a cap smaller than the plain
+// limit walks once more for them, uncapped (persistent waves take new units meanwhile, so the retries do
+// not form a tail of their own as a separate launch over scattered queries would).
+template <int D, bool CAPPED>
 __global__ __launch_bounds__(kBlock) void kBatch(const BatchItem* __restrict__ items, int nItems, int64_t total,
                                                  DistOpts o, float up, unsigned long long* __restrict__ tickets,
                                                  int xcdSplit)
@@ -368,29 +371,23 @@ __global__ __launch_bounds__(kBlock) void kBatch(const BatchItem* __restrict__ i
         const int64_t lu = u - it.unitBegin;
         if (lu < it.gridUnits) {
             int x = 0, y = 0, z = 0;
-            bool valid;
-            size_t idx = 0;
-            if (it.voxList) {
-                const int64_t i = lu * 64 + lane;
-                valid = i < it.nVoxList;
-                if (valid) {
-                    idx = it.voxList[i];
-                    x = (int)(idx % (size_t)it.dx);
-                    y = (int)((idx / (size_t)it.dx) % (size_t)it.dy);
-                    z = (int)(idx / ((size_t)it.dx * it.dy));
-                }
-            } else {
-                valid = brickVoxel((int)lu, lane, it.nbx, it.nby, it.nbz, it.dx, it.dy, 0, it.dz, x, y, z);
-                idx = valid ? ((size_t)z * it.dy + y) * it.dx + x : 0;
-            }
+            const bool valid = brickVoxel((int)lu, lane, it.nbx, it.nby, it.nbz, it.dx, it.dy, 0, it.dz, x, y, z);
+            const size_t idx = valid ? ((size_t)z * it.dy + y) * it.dx + x : 0;
             float px = (float)x, py = (float)y, pz = (float)z;
             xfApply(it.xf, px, py, pz);
             float lim = o.maxDistSq;
-            if (it.capSdf && valid) { const float c = it.capSdf[idx]; lim = c < lim ? c : lim; }
-            const float resSq = coopDistance<D>(it.m, px, py, pz, valid ? lim : -1.f, &stkMem[w].dist);
-            const bool again = it.rerunVox && valid && !(resSq < lim) && lim < o.maxDistSq;
-            if (it.rerunVox) appendWave(again, (uint32_t)idx, it.rerunVox, it.rerunCount);
-            sdfStore(it.m, px, py, pz, valid && !again, resSq, o, stkMem[w].fwn, it.sdf, idx);
+            if (CAPPED && it.capSdf && valid) { const float c = it.capSdf[idx]; lim = c < lim ? c : lim; }
+            float resSq = coopDistance<D>(it.m, px, py, pz, valid ? lim : -1.f, &stkMem[w].dist);
+            if (CAPPED) {
+                const bool retry = valid && !(resSq < lim) && lim < o.maxDistSq;
+                const uint64_t rm = waveBallot(retry);
+                if (rm) {
+                    if (lane == 0) atomicAdd(it.rerunCount, (uint32_t)__popcll(rm));
+                    const float r = coopDistance<D>(it.m, px, py, pz, retry ? o.maxDistSq : -1.f, &stkMem[w].dist);
+                    if (retry) resSq = r;
+                }
+            }
+            sdfStore(it.m, px, py, pz, valid, resSq, o, stkMem[w].fwn, it.sdf, idx);
         } else {
             const int64_t i = (lu - it.gridUnits) * 64 + lane;
             const bool valid = i < it.npts;
@@ -399,13 +396,21 @@ __global__ __launch_bounds__(kBlock) void kBatch(const BatchItem* __restrict__ i
             float lim = up;
             if (valid) {
                 px = it.pts[3 * q]; py = it.pts[3 * q + 1]; pz = it.pts[3 * q + 2];
-                if (it.capProj) { const float c = it.capProj[q]; lim = c < up ? c : up; }
+                if (CAPPED && it.capProj) { const float c = it.capProj[q]; lim = c < up ? c : up; }
             }
             BestHit h;
             coopProject<D>(it.m, px, py, pz, lim, valid, &stkMem[w].proj, h);
-            const bool again = it.rerunPts && valid && h.face < 0 && lim < up;
-            if (it.rerunPts) appendWave(again, (uint32_t)q, it.rerunPts, it.rerunCount + 1);
-            if (valid && !again) {
+            if (CAPPED) {
+                const bool retry = valid && h.face < 0 && lim < up;
+                const uint64_t rm = waveBallot(retry);
+                if (rm) {
+                    if (lane == 0) atomicAdd(it.rerunCount + 1, (uint32_t)__popcll(rm));
+                    BestHit h2;
+                    coopProject<D>(it.m, px, py, pz, up, retry, &stkMem[w].proj, h2);
+                    if (retry) h = h2;
+                }
+            }
+            if (valid) {
                 float4* op = reinterpret_cast<float4*>(it.proj + q);
                 op[0] = make_float4(__int_as_float(h.face), h.px, h.py, h.pz);
                 op[1] = make_float4(__int_as_float(-1), h.a, h.b, h.distSq);
@@ -585,14 +590,19 @@ __global__ void kFillU32(uint32_t* p, int n, uint32_t v)
 } // namespace
 
 hipError_t launchBatch(const BatchItem* items, int nItems, int64_t totalUnits, const DistOpts& o, float up,
-                       unsigned long long* tickets, int xcdSplit, int blocks, int stackD, hipStream_t s)
+                       unsigned long long* tickets, int xcdSplit, int blocks, int stackD, bool capped,
+                       hipStream_t s)
 {
     if (totalUnits <= 0 || nItems <= 0) return hipSuccess;
     if (o.minDistSq > 0 || blocks <= 0) return hipErrorInvalidValue;
     return dispatchDepth(stackD, [&](auto Dc) {
         constexpr int D = decltype(Dc)::value;
-        hipLaunchKernelGGL(kBatch<D>, dim3(blocks), dim3(kBlock), 0, s, items, nItems, totalUnits, o, up, tickets,
-                           xcdSplit);
+        if (capped)
+            hipLaunchKernelGGL((kBatch<D, true>), dim3(blocks), dim3(kBlock), 0, s, items, nItems, totalUnits, o, up,
+                               tickets, xcdSplit);
+        else
+            hipLaunchKernelGGL((kBatch<D, false>), dim3(blocks), dim3(kBlock), 0, s, items, nItems, totalUnits, o, up,
+                               tickets, xcdSplit);
         return hipGetLastError();
     });
 }

@@ -96,23 +96,19 @@ struct BatchItem {
     int64_t npts;
     ProjRec* proj;
     int64_t gridUnits, projUnits, unitBegin;
-    // list mode: the grid part covers only these voxel ids (64 per unit) instead of all bricks
-    const uint32_t* voxList;
-    int64_t nVoxList;
-    // optional per-query caps (squared) on the search radius, indexed like the outputs; with caps,
-    // a query that found nothing below its (smaller than the plain limit) cap is not written but
-    // appended to rerunVox / rerunPts, counted in rerunCount[0] / [1]
+    // capped launch only: optional per-query caps (squared) on the search radius, indexed like the
+    // outputs; a wave walks once more, uncapped, for its queries that found nothing below a cap smaller
+    // than the plain limit, counted in rerunCount[0] (voxels) / [1] (points)
     const float* capSdf;
     const float* capProj;
-    uint32_t* rerunVox;
-    uint32_t* rerunPts;
     uint32_t* rerunCount;
 };
 
 // persistent launch over items[0..nItems) (device array); tickets = 8 zeroed 64-bit counters;
-// blocks = number of 256-thread workgroups to keep resident
+// blocks = number of 256-thread workgroups to keep resident; capped: the items' caps apply
 hipError_t launchBatch(const BatchItem* items, int nItems, int64_t totalUnits, const DistOpts& o, float up,
-                       unsigned long long* tickets, int xcdSplit, int blocks, int stackD, hipStream_t s);
+                       unsigned long long* tickets, int xcdSplit, int blocks, int stackD, bool capped,
+                       hipStream_t s);
 
 // small fill kernel (used instead of hipMemsetAsync so streams stay graph-capture safe)
 // waves of the wave-cooperative signed-distance launch over z-layers [zBegin, zEnd)
