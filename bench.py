@@ -581,6 +581,7 @@ def summarize_pmc(src_dir, name):
     counter sums per kernel name over its dispatches, the 1 GiB copy's readings as the byte calibration,
     and the hash of the kernel sources the profile was taken on."""
     kernels = {}
+    copy_max = {}      # the largest single device-copy dispatch per counter: the 1 GiB calibration copy
     for fn in sorted(glob.glob(os.path.join(src_dir, "**", "*counter_collection.csv"), recursive=True)):
         seen = {}
         with open(fn, newline="") as fh:
@@ -598,15 +599,13 @@ def summarize_pmc(src_dir, name):
                 k = kernels.setdefault(short, {})
                 k[cname] = k.get(cname, 0.0) + val
                 seen.setdefault(short, set()).add(disp)
+                if "copyBuffer" in short:
+                    copy_max[cname] = max(copy_max.get(cname, 0.0), val)
         for short, ds in seen.items():
             kernels[short]["dispatches"] = max(kernels[short].get("dispatches", 0), len(ds))
-    copy = next((k for n, k in kernels.items() if "copyBuffer" in n and k.get("FETCH_SIZE")), None)
-    copy_w = next((k for n, k in kernels.items() if "copyBuffer" in n and k.get("WRITE_SIZE")), None)
-    if copy is None or copy_w is None:
-        raise SystemExit("no device copy with FETCH_SIZE / WRITE_SIZE in the profile (run with --copy-calibration)")
-    # the largest copy dispatch is the 1 GiB calibration copy; other copies are small
-    cal = {"copy_bytes": COPY_BYTES, "fetch_kib": copy["FETCH_SIZE"] / max(copy.get("dispatches", 1), 1),
-           "write_kib": copy_w["WRITE_SIZE"] / max(copy_w.get("dispatches", 1), 1)}
+    if not copy_max.get("FETCH_SIZE") or not copy_max.get("WRITE_SIZE"):
+        raise SystemExit("no device copy with FETCH_SIZE / WRITE_SIZE in the profile (run with --profile-run)")
+    cal = {"copy_bytes": COPY_BYTES, "fetch_kib": copy_max["FETCH_SIZE"], "write_kib": copy_max["WRITE_SIZE"]}
     out = {"name": name, "sources_sha256": sources_sha256(), "calibration": cal,
            "kernels": {k: v for k, v in kernels.items() if not k.startswith("__amd")}}
     path = os.path.join(ROOT, "profiles", f"pmc_{name}.json")
