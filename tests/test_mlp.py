@@ -114,3 +114,46 @@ def test_gpu_training_reduces_loss_and_bound_is_exact():
     fwn.sdf_grid_learned_device(net, 0.1, dims, xf, M.DistanceToMeshOptions(), dsdf.data_ptr())
     torch.cuda.synchronize()
     assert np.array_equal(dsdf.cpu().numpy().view(np.uint32), plain.view(np.uint32))
+
+
+@pytest.mark.gpu
+def test_gpu_calibration_quantile_and_capped_reruns():
+    """The slack is the requested quantile of label - prediction (never below 0), computed from the
+    network's own predictions; with caps (pred (1 + m) + slack)^2 the projections that exceed their cap
+    are exactly those the call reports as reruns, and the output stays exact."""
+    import torch
+    M = _mlp_module()
+    from meshlib_amd import synth
+    v, f = synth.bumpy_icosphere(4, 6, 3)
+    mesh = M.Mesh(v, f)
+    pr = M.PointsToMeshProjector()
+    pr.updateMeshData(mesh)
+    lo, hi = v.min(0), v.max(0)
+    net = M.DistanceMLP(0.5 * (lo + hi), float(0.5 * np.max(hi - lo) * 1.2), seed=3)
+    pts = synth.projection_queries(v, f, 20000, 5)
+    dp = torch.from_numpy(pts).cuda()
+    pred = net.predict(dp).astype(np.float64)
+    # labels = prediction + a known excess per point: the slack is that excess's quantile
+    rng = np.random.default_rng(9)
+    excess = rng.uniform(-0.05, 0.2, size=pred.shape[0])
+    lab = torch.from_numpy((pred + excess).astype(np.float32)).cuda()
+    for q in (0.5, 0.9, 0.99):
+        got = net.calibrate(dp, lab, q)
+        under = np.sort((lab.cpu().numpy().astype(np.float64) - pred).astype(np.float32))
+        want = max(0.0, float(under[min(len(under) - 1, int(np.floor(q * (len(under) - 1) + 0.5)))]))
+        assert abs(got - want) <= 1e-6 * max(1.0, abs(want)), (q, got, want)
+    # all labels under the prediction: slack 0
+    assert net.calibrate(dp, torch.from_numpy((0.5 * pred).astype(np.float32)).cuda(), 0.99) == 0.0
+    # caps with a slack: exact records, reruns = points whose true distance reaches the cap
+    rec = torch.empty(pts.shape[0] * 8, dtype=torch.int32, device="cuda")
+    pr.find_projections_device(dp.data_ptr(), pts.shape[0], rec.data_ptr())
+    true_d = rec.view(torch.float32).view(-1, 8)[:, 7].cpu().numpy()
+    for slack in (0.0, 0.02):
+        net.set_slack(slack)
+        out = torch.empty_like(rec)
+        reruns = pr.find_projections_learned_device(net, 0.1, dp.data_ptr(), pts.shape[0], out.data_ptr())
+        torch.cuda.synchronize()
+        assert torch.equal(out, rec), slack
+        cap = (pred.astype(np.float32) * np.float32(1.1) + np.float32(slack)) ** 2
+        expect = int(np.sum(~(true_d < cap)))
+        assert abs(reruns - expect) <= max(2, expect // 1000), (slack, reruns, expect)
