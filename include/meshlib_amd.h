@@ -4,8 +4,9 @@
  * Plain pointers and sizes, no C++ or torch types.  Every entry point returns
  * MI_OK (0) on success or a negative MI_E_* status; mi_last_error() returns the
  * calling thread's last message.  Host-pointer entry points behave like the
- * reference's synchronous calls (upload, compute, download; chunked to fit
- * device memory).  *_device entry points take device pointers that are already
+ * reference's synchronous calls (upload, compute, download, within the device
+ * memory budget); they run in parts (64 MiB of voxels, 2^21 points) whose copies
+ * overlap the computation of the next part.  *_device entry points take device pointers that are already
  * resident in HBM and an optional hipStream_t (NULL = default stream); they are
  * asynchronous with respect to the host.
  *
@@ -222,7 +223,10 @@ int mi_fwn_calc_from_grid_with_distances(mi_mesh* m, const int32_t dims[3], cons
                                          const mi_dist_options* opts, float* res,
                                          mi_progress_fn cb, void* user);
 /* IFastWindingNumberByParts::calcFromGridByParts / calcFromGridWithDistancesByParts
- * (MRFastWindingNumber.h:96-106; GPU MRCudaFastWindingNumber.cpp:285-429) */
+ * (MRFastWindingNumber.h:96-106; GPU MRCudaFastWindingNumber.cpp:285-429).  Parts are whole
+ * z-layers, about 64 MiB of output each (at least 4 * layerOverlap layers, at most half the memory
+ * budget), consecutive parts sharing layerOverlap layers; fn gets a page-locked buffer that is
+ * reused once it returns, so it copies what it keeps. */
 int mi_fwn_calc_from_grid_by_parts(mi_mesh* m, mi_grid_part_fn fn, void* fnUser, const int32_t dims[3],
                                    const float xf[12], float beta, int32_t layerOverlap,
                                    mi_progress_fn cb, void* user);

@@ -232,6 +232,54 @@ def test_chunked_and_by_parts(k5):
         M.set_memory_limit(0)
 
 
+def test_part_pipelines_at_full_budget(k5):
+    """Host-pointer and by-parts calls larger than one pipeline part (64 MiB of voxels, 2^21 points):
+    several parts in flight on two streams equal the device entry points bit for bit."""
+    import torch
+    v, f, gm, om = k5
+    fwn = M.FastWindingNumber(gm)
+    opts = M.DistanceToMeshOptions()
+    dims, xf, vox = synth.grid_for_mesh(v, (256, 256, 300))   # 256 + 44 layers
+    nvox, layer = 256 * 256 * 300, 256 * 256
+    dev = torch.empty(nvox, dtype=torch.float32, device="cuda")
+    for name, host, fill in (
+            ("sdf", lambda: fwn.calcFromGridWithDistances(dims, xf, opts),
+             lambda: fwn.sdf_grid_device(dims, xf, opts, dev.data_ptr())),
+            ("fwn", lambda: fwn.calcFromGrid(dims, xf, 2.0),
+             lambda: fwn.fwn_grid_device(dims, xf, 2.0, dev.data_ptr()))):
+        say("host-pointer grid", name)
+        fill()
+        torch.cuda.synchronize()
+        assert np.array_equal(_bits(host()), _bits(dev.cpu().numpy())), name
+    say("by parts, overlap 1")
+    fwn.sdf_grid_device(dims, xf, opts, dev.data_ptr())
+    full = dev.cpu().numpy()
+    parts = []
+    fwn.calcFromGridWithDistancesByParts(lambda data, d, z0: parts.append((z0, d, data)), dims, xf, opts, 1)
+    assert len(parts) == 2 and parts[1][0] == parts[0][1][2] - 1
+    for z0, d, data in parts:
+        assert np.array_equal(_bits(data), _bits(full[z0 * layer:(z0 + d[2]) * layer]))
+    assert parts[-1][0] + parts[-1][1][2] == 300
+
+    n = (2 << 21) + 4321                                      # two full parts and a ragged one
+    pts = synth.projection_queries(v, f, n, 12)
+    dp = torch.from_numpy(pts).cuda()
+    pr = M.PointsToMeshProjector()
+    pr.updateMeshData(gm)
+    say("host-pointer projections", n)
+    dr = torch.empty(n * 8, dtype=torch.int32, device="cuda")
+    pr.find_projections_device(dp.data_ptr(), n, dr.data_ptr())
+    torch.cuda.synchronize()
+    assert np.array_equal(_bits(pr.findProjections(pts)), _bits(dr.cpu().numpy()))
+    say("host-pointer winding numbers", n)
+    dw = torch.empty(n, dtype=torch.float32, device="cuda")
+    fwn.fwn_vector_device(dp.data_ptr(), n, 2.0, dw.data_ptr())
+    torch.cuda.synchronize()
+    assert np.array_equal(_bits(fwn.calcFromVector(pts, 2.0)), _bits(dw.cpu().numpy()))
+    sel = np.random.default_rng(3).choice(n, 3000, replace=False)
+    assert np.array_equal(_bits(dr.cpu().numpy().view(M.PROJ_DTYPE)[sel]), _bits(om.find_projections(pts[sel])))
+
+
 def test_device_entry_points_match_host(k5):
     import torch
     v, f, gm, om = k5
