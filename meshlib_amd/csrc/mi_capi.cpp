@@ -946,34 +946,33 @@ static int gridLaunch(mi_mesh* m, const int32_t dims[3], const mi::Xf& x, const 
     return MI_OK;
 }
 
-// side stream and per-slot events of a part pipeline, released (after both streams drain) on scope exit
+// The two streams of a part pipeline (the mesh's stream and a second one) and its events; both
+// streams drain before the second stream and the events are released.
 struct PartStreams {
-    hipStream_t main = nullptr, side = nullptr;
-    std::vector<hipEvent_t> uploaded, computed, copied;
+    hipStream_t st[2] = { nullptr, nullptr };
+    std::vector<hipEvent_t> ev;
     ~PartStreams()
     {
-        if (main) (void)hipStreamSynchronize(main);
-        if (side) { (void)hipStreamSynchronize(side); (void)hipStreamDestroy(side); }
-        for (auto* v : { &uploaded, &computed, &copied })
-            for (hipEvent_t e : *v) if (e) (void)hipEventDestroy(e);
+        if (st[0]) (void)hipStreamSynchronize(st[0]);
+        if (st[1]) { (void)hipStreamSynchronize(st[1]); (void)hipStreamDestroy(st[1]); }
+        for (hipEvent_t e : ev) if (e) (void)hipEventDestroy(e);
     }
-    hipError_t init(hipStream_t mainStream, size_t slots)
+    hipError_t init(hipStream_t mainStream, size_t events = 0)
     {
-        main = mainStream;
-        hipError_t e = hipStreamCreateWithFlags(&side, hipStreamNonBlocking);
-        if (e != hipSuccess) { side = nullptr; return e; }
-        for (auto* v : { &uploaded, &computed, &copied }) {
-            v->assign(slots, nullptr);
-            for (hipEvent_t& ev : *v)
-                if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) return e;
-        }
+        st[0] = mainStream;
+        hipError_t e = hipStreamCreateWithFlags(&st[1], hipStreamNonBlocking);
+        if (e != hipSuccess) { st[1] = nullptr; return e; }
+        ev.assign(events, nullptr);
+        for (hipEvent_t& x : ev)
+            if ((e = hipEventCreateWithFlags(&x, hipEventDisableTiming)) != hipSuccess) return e;
         return hipSuccess;
     }
 };
 
-// The volume is computed in parts of whole z-layers into a ring of device slots (as many as the memory
-// budget holds, maxBufferSizeAlignedByBlock, source/MRCuda/MRCudaBasic.cpp:103-107); each part is copied
-// to the caller's array on a side stream while the next parts compute.
+// The volume is computed in parts of whole z-layers, all queued on the mesh's stream into a ring of
+// device slots (as many as the memory budget holds, maxBufferSizeAlignedByBlock,
+// source/MRCuda/MRCudaBasic.cpp:103-107); a second stream copies each finished part to the caller's
+// array while the later parts compute, and a slot takes its next part once its copy is done.
 static int gridHost(mi_mesh* m, const int32_t dims[3], const float xf[12], const mi_dist_options* opts, float beta,
                     float* res, mi_progress_fn cb, void* user)
 {
@@ -998,17 +997,19 @@ static int gridHost(mi_mesh* m, const int32_t dims[3], const float xf[12], const
     mi::SplitScratch sc;
     if (opts)
         if (int rc = splitScratch(m, splitBuf, dims, (int32_t)part, sdfLaneBytes(opts), sc)) return rc;
-    PartStreams ps;
-    MI_HIP(ps.init(m->stream, slots));
+    PartStreams ps;   // st[0] computes, st[1] copies; events: computed / copied per slot
+    MI_HIP(ps.init(m->stream, 2 * slots));
+    hipEvent_t* const computed = ps.ev.data();
+    hipEvent_t* const copied = ps.ev.data() + slots;
     if (!report(cb, user, 0.6f)) return fail(MI_E_CANCELED, "Operation was canceled");
     const mi::Xf x = toXf(xf);
     float* const ring = out.as<float>();
     auto launch = [&](size_t k) -> int {
         const size_t slot = k % slots;
-        if (k >= slots) MI_HIP(hipStreamWaitEvent(ps.main, ps.copied[slot], 0));   // the slot's last copy
+        if (k >= slots) MI_HIP(hipStreamWaitEvent(ps.st[0], copied[slot], 0));   // the slot's last copy
         const int32_t z0 = (int32_t)(k * part), z1 = (int32_t)std::min(nz, (k + 1) * part);
-        if (int rc = gridLaunch(m, dims, x, opts, beta, z0, z1, ring + slot * part * layer, ps.main, &sc)) return rc;
-        MI_HIP(hipEventRecord(ps.computed[slot], ps.main));
+        if (int rc = gridLaunch(m, dims, x, opts, beta, z0, z1, ring + slot * part * layer, ps.st[0], &sc)) return rc;
+        MI_HIP(hipEventRecord(computed[slot], ps.st[0]));
         return MI_OK;
     };
     for (size_t k = 0; k < slots; ++k)
@@ -1016,13 +1017,13 @@ static int gridHost(mi_mesh* m, const int32_t dims[3], const float xf[12], const
     for (size_t k = 0; k < nParts; ++k) {
         const size_t slot = k % slots;
         const size_t z0 = k * part, z1 = std::min(nz, (k + 1) * part);
-        MI_HIP(hipStreamWaitEvent(ps.side, ps.computed[slot], 0));
+        MI_HIP(hipStreamWaitEvent(ps.st[1], computed[slot], 0));
         MI_HIP(hipMemcpyAsync(res + z0 * layer, ring + slot * part * layer, (z1 - z0) * layer * sizeof(float),
-                              hipMemcpyDeviceToHost, ps.side));
-        MI_HIP(hipEventRecord(ps.copied[slot], ps.side));
+                              hipMemcpyDeviceToHost, ps.st[1]));
+        MI_HIP(hipEventRecord(copied[slot], ps.st[1]));
         if (k + slots < nParts)
             if (int rc = launch(k + slots)) return rc;
-        MI_HIP(hipStreamSynchronize(ps.side));
+        MI_HIP(hipStreamSynchronize(ps.st[1]));
         if (!report(cb, user, 0.6f + 0.4f * (float)(k + 1) / (float)nParts))
             return fail(MI_E_CANCELED, "Operation was canceled");
     }
@@ -1042,7 +1043,7 @@ int mi_fwn_calc_from_grid_with_distances(mi_mesh* m, const int32_t dims[3], cons
     return gridHost(m, dims, xf, opts, 0.f, res, cb, user);
 }
 
-// by-parts: the GPU computes part k+1 while the caller's callback consumes part k
+// by-parts: the GPU computes parts k+1 (and k+2) while the caller's callback consumes part k
 // (cudaPipeline, source/MRCuda/MRCudaPipeline.hpp:9-48, on two HIP streams). Parts are about
 // kPartBytes of output (at least 4 overlaps long, at most half the memory budget); their boundaries
 // follow splitByChunks (source/MRMesh/MRChunkIterator.cpp:29-53) in units of z-layers.
@@ -1072,46 +1073,46 @@ static int gridByParts(mi_mesh* m, mi_grid_part_fn fn, void* fnUser, const int32
         step = chunkLayers - (size_t)layerOverlap;
         nChunks = (nz - (size_t)layerOverlap + step - 1) / step;
     }
+    const size_t slots = std::min<size_t>(2, nChunks);
     DevBuf dbuf[2], splitBuf[2];
     mi::SplitScratch sc[2];
-    if (opts)
-        for (int i = 0; i < 2; ++i)
-            if (int rc = splitScratch(m, splitBuf[i], dims, (int32_t)chunkLayers, sdfLaneBytes(opts), sc[i])) return rc;
     PinnedBuf hbuf[2];
-    PartStreams ps;
-    MI_HIP(ps.init(m->stream, 2));
-    hipStream_t st[2] = { ps.main, ps.side };
-    for (int i = 0; i < 2; ++i) {
+    for (size_t i = 0; i < slots; ++i) {
         MI_HIP(dbuf[i].alloc(chunkLayers * layer * sizeof(float)));
         MI_HIP(hbuf[i].take(chunkLayers * layer * sizeof(float)));
+        if (opts)
+            if (int rc = splitScratch(m, splitBuf[i], dims, (int32_t)chunkLayers, sdfLaneBytes(opts), sc[i])) return rc;
     }
+    PartStreams ps;
+    MI_HIP(ps.init(m->stream));
     if (!report(cb, user, 0.6f)) return fail(MI_E_CANCELED, "Operation was canceled");
     const mi::Xf x = toXf(xf);
     auto chunkRange = [&](size_t c, int32_t& z0, int32_t& z1) {
         z0 = (int32_t)(c * step);
         z1 = (int32_t)std::min(nz, (size_t)z0 + chunkLayers);
     };
+    // part c into slot c % 2: compute, then copy to the slot's page-locked buffer, on the slot's stream
     auto launch = [&](size_t c) -> int {
-        const int b = (int)(c & 1);
+        const size_t b = c % slots;
         int32_t z0, z1;
         chunkRange(c, z0, z1);
-        if (int rc = gridLaunch(m, dims, x, opts, beta, z0, z1, dbuf[b].as<float>(), st[b], &sc[b])) return rc;
+        if (int rc = gridLaunch(m, dims, x, opts, beta, z0, z1, dbuf[b].as<float>(), ps.st[b], &sc[b])) return rc;
         MI_HIP(hipMemcpyAsync(hbuf[b].p, dbuf[b].p, (size_t)(z1 - z0) * layer * sizeof(float),
-                              hipMemcpyDeviceToHost, st[b]));
-        MI_HIP(hipEventRecord(ps.computed[b], st[b]));
+                              hipMemcpyDeviceToHost, ps.st[b]));
         return MI_OK;
     };
-    if (int rc = launch(0)) return rc;
+    for (size_t c = 0; c < slots; ++c)
+        if (int rc = launch(c)) return rc;
     for (size_t c = 0; c < nChunks; ++c) {
-        const int b = (int)(c & 1);
-        MI_HIP(hipEventSynchronize(ps.computed[b]));
-        if (c + 1 < nChunks)
-            if (int rc = launch(c + 1)) return rc;
+        const size_t b = c % slots;
+        MI_HIP(hipStreamSynchronize(ps.st[b]));
         int32_t z0, z1;
         chunkRange(c, z0, z1);
         const int32_t pd[3] = { dims[0], dims[1], z1 - z0 };
         const int rc = fn(hbuf[b].as<float>(), pd, z0, fnUser);
         if (rc != MI_OK) return fail(rc, "grid part callback failed");
+        if (c + slots < nChunks)   // the slot's buffers are free again
+            if (int rc2 = launch(c + slots)) return rc2;
         if (!report(cb, user, 0.6f + 0.4f * (float)(c + 1) / (float)nChunks))
             return fail(MI_E_CANCELED, "Operation was canceled");
     }
@@ -1138,8 +1139,8 @@ int mi_fwn_calc_from_grid_with_distances_by_parts(mi_mesh* m, mi_grid_part_fn fn
 
 // Per-point pipeline shared by calcFromVector and findProjections: points go in parts of up to
 // kPartPoints (within the memory budget, maxBufferSize, source/MRCuda/MRCudaBasic.cpp:94-97, with
-// the sort workspace counted); a part is uploaded and its results downloaded on a side stream while
-// the part before it is sorted and walked on the mesh's stream.
+// the sort workspace counted), two in flight on two streams; each part is uploaded, sorted, walked
+// and downloaded on its slot's stream while the other slot's part runs.
 constexpr size_t kPartPoints = size_t(1) << 21;
 
 template <class LaunchFn>
@@ -1155,7 +1156,7 @@ static int pointsHost(mi_mesh* m, const float* pts, int64_t n, size_t outElem, v
     const size_t perElem = 12 + outElem + 20 + (split ? 9 : 0);
     const size_t budget = memBudget() / perElem;
     if (budget == 0) return fail(MI_E_LIMIT, "no point fits in the device memory budget");
-    const size_t part = std::min({ budget, kPartPoints, (size_t)n });
+    const size_t part = std::min({ kPartPoints, (size_t)n, std::max<size_t>(1, budget / 2) });
     const size_t nParts = ((size_t)n + part - 1) / part;
     const size_t slots = std::min({ (size_t)2, budget / part, nParts });
     struct Slot {
@@ -1172,25 +1173,22 @@ static int pointsHost(mi_mesh* m, const float* pts, int64_t n, size_t outElem, v
         }
     }
     PartStreams ps;
-    MI_HIP(ps.init(m->stream, slots));
+    MI_HIP(ps.init(m->stream));
     if (!report(cb, user, 0.6f)) return fail(MI_E_CANCELED, "Operation was canceled");
     auto range = [&](size_t k, size_t& off, size_t& sz) {
         off = k * part;
         sz = std::min(part, (size_t)n - off);
     };
-    // upload part k (the side stream has already downloaded the slot's previous part) and walk it
     auto start = [&](size_t k) -> int {
         Slot& s = sl[k % slots];
+        const hipStream_t st = ps.st[k % slots];
         size_t off, sz;
         range(k, off, sz);
-        MI_HIP(hipMemcpyAsync(s.pts.p, pts + 3 * off, sz * 12, hipMemcpyHostToDevice, ps.side));
-        MI_HIP(hipEventRecord(ps.uploaded[k % slots], ps.side));
-        MI_HIP(hipStreamWaitEvent(ps.main, ps.uploaded[k % slots], 0));
+        MI_HIP(hipMemcpyAsync(s.pts.p, pts + 3 * off, sz * 12, hipMemcpyHostToDevice, st));
         float* const dp = static_cast<float*>(s.pts.p);
         const uint32_t* perm = nullptr;
-        MI_HIP(mi::sortPointsMorton(dp, (int64_t)sz, m->sortLo, m->sortHi, s.ws.p, s.ws.bytes, &perm, ps.main));
-        MI_HIP(launchFn(dp, perm, (int64_t)sz, s.res.p, &s.sc));
-        MI_HIP(hipEventRecord(ps.computed[k % slots], ps.main));
+        MI_HIP(mi::sortPointsMorton(dp, (int64_t)sz, m->sortLo, m->sortHi, s.ws.p, s.ws.bytes, &perm, st));
+        MI_HIP(launchFn(dp, perm, (int64_t)sz, s.res.p, &s.sc, st));
         return MI_OK;
     };
     for (size_t k = 0; k < slots; ++k)
@@ -1198,12 +1196,12 @@ static int pointsHost(mi_mesh* m, const float* pts, int64_t n, size_t outElem, v
     for (size_t k = 0; k < nParts; ++k) {
         size_t off, sz;
         range(k, off, sz);
-        MI_HIP(hipStreamWaitEvent(ps.side, ps.computed[k % slots], 0));
+        const hipStream_t st = ps.st[k % slots];
         MI_HIP(hipMemcpyAsync(static_cast<char*>(res) + off * outElem, sl[k % slots].res.p, sz * outElem,
-                              hipMemcpyDeviceToHost, ps.side));
+                              hipMemcpyDeviceToHost, st));
+        MI_HIP(hipStreamSynchronize(st));
         if (k + slots < nParts)
             if (int rc = start(k + slots)) return rc;
-        MI_HIP(hipStreamSynchronize(ps.side));
         if (!report(cb, user, 0.6f + 0.4f * (float)(k + 1) / (float)nParts))
             return fail(MI_E_CANCELED, "Operation was canceled");
     }
@@ -1217,9 +1215,10 @@ int mi_fwn_calc_from_vector(mi_mesh* m, const float* pts, int64_t n, float beta,
 {
     if (!m || n < 0 || (n > 0 && (!pts || !res))) return fail(MI_E_INVALID, "invalid arguments");
     return pointsHost(m, pts, n, sizeof(float), res,
-                      [&](const float* dp, const uint32_t* perm, int64_t sz, void* dr, const mi::SplitScratch*) {
+                      [&](const float* dp, const uint32_t* perm, int64_t sz, void* dr, const mi::SplitScratch*,
+                          hipStream_t st) {
                           return mi::launchFwnPoints(callView(m), dp, perm, sz, beta, skipFace, static_cast<float*>(dr),
-                                                     m->stackD, m->stream);
+                                                     m->stackD, st);
                       },
                       cb, user);
 }
@@ -1264,10 +1263,11 @@ int mi_proj_find_projections(mi_mesh* m, const float* pts, int64_t n, const floa
     if (treeXf) tx = toXf(treeXf);
     (void)kIdentityXf;
     return pointsHost(m, pts, n, sizeof(mi_projection), res,
-                      [&](const float* dp, const uint32_t* perm, int64_t sz, void* dr, const mi::SplitScratch* sc) {
+                      [&](const float* dp, const uint32_t* perm, int64_t sz, void* dr, const mi::SplitScratch* sc,
+                          hipStream_t st) {
                           return mi::launchProjections(callView(m), dp, perm, sz, ptXf ? &px : nullptr,
                                                        treeXf ? &tx : nullptr, up, lo,
-                                                       static_cast<mi::ProjRec*>(dr), m->stackD, m->stream,
+                                                       static_cast<mi::ProjRec*>(dr), m->stackD, st,
                                                        nullptr, nullptr, nullptr, sc);
                       },
                       nullptr, nullptr, true);
