@@ -235,8 +235,6 @@ mi::Xf toXf(const float* xf)
     return x;
 }
 
-const float kIdentityXf[12] = { 1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0 };
-
 } // namespace
 
 // error reporting shared with the other translation units of the library
@@ -1261,7 +1259,6 @@ int mi_proj_find_projections(mi_mesh* m, const float* pts, int64_t n, const floa
     mi::Xf px, tx;
     if (ptXf) px = toXf(ptXf);
     if (treeXf) tx = toXf(treeXf);
-    (void)kIdentityXf;
     return pointsHost(m, pts, n, sizeof(mi_projection), res,
                       [&](const float* dp, const uint32_t* perm, int64_t sz, void* dr, const mi::SplitScratch* sc,
                           hipStream_t st) {

@@ -117,3 +117,55 @@ def test_mesh_mode_rejects_bad_values():
         m.set_mode("traversal", "sideways")
     rc = M.lib().mi_mesh_set_mode(m.handle, 9, 0)
     assert rc == M.MI_E_INVALID
+
+
+def test_threads_on_meshes_with_their_own_settings():
+    """Four host threads, each on its own mesh with its own settings (per-lane / wave-cooperative,
+    split form on / off, leaf filter on / off), run host-pointer signed distance, projections and
+    by-parts calls at the same time (the library releases no lock across meshes, and by-parts
+    callers share the pool of page-locked buffers): every result equals the oracle."""
+    from concurrent.futures import ThreadPoolExecutor
+    v, f = synth.bumpy_icosphere(5, 8, 4)
+    dims, xf, vox = synth.grid_for_mesh(v, (48, 40, 44))
+    pts = synth.projection_queries(v, f, 30000, 9)
+    om = O.Mesh(v, f)
+    ref_sdf = om.sdf_grid(dims, xf)
+    ref_proj = om.find_projections(pts)
+    far = np.abs(ref_sdf) > 1e-3 * float(vox.min())
+    settings = [{"traversal": "per_lane"},
+                {"traversal": "wave_coop", "split_sdf": 0, "split_projections": 0},
+                {"traversal": "wave_coop", "split_sdf": 1, "split_projections": 1, "leaf_filter": "off"},
+                {"traversal": "wave_coop", "leaf_filter": "on"}]
+    meshes = []
+    for s in settings:
+        m = M.Mesh(v, f)
+        for k, val in s.items():
+            m.set_mode(k, val)
+        pr = M.PointsToMeshProjector()
+        pr.updateMeshData(m)
+        meshes.append((m, M.FastWindingNumber(m), pr))
+    layer = dims[0] * dims[1]
+
+    def work(i):
+        _, fwn, pr = meshes[i]
+        errs = []
+        for rep in range(3):
+            g = fwn.calcFromGridWithDistances(dims, xf, M.DistanceToMeshOptions())
+            if not (np.array_equal(np.abs(g).view(np.uint32), np.abs(ref_sdf).view(np.uint32))
+                    and np.array_equal(np.signbit(g[far]), np.signbit(ref_sdf[far]))):
+                errs.append(("sdf", rep))
+            if not np.array_equal(pr.findProjections(pts).view(np.uint8), ref_proj.view(np.uint8)):
+                errs.append(("projections", rep))
+            got = np.full(ref_sdf.size, np.nan, np.float32)
+
+            def take(data, d, z0):
+                got[z0 * layer:(z0 + d[2]) * layer] = data
+            fwn.calcFromGridWithDistancesByParts(take, dims, xf, M.DistanceToMeshOptions(), 1)
+            if not np.array_equal(got.view(np.uint32), g.view(np.uint32)):
+                errs.append(("by parts", rep))
+        return errs
+
+    with ThreadPoolExecutor(len(meshes)) as ex:
+        results = list(ex.map(work, range(len(meshes))))
+    say("per-thread mismatches:", results)
+    assert all(not r for r in results)

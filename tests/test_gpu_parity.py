@@ -279,6 +279,23 @@ def test_part_pipelines_at_full_budget(k5):
     sel = np.random.default_rng(3).choice(n, 3000, replace=False)
     assert np.array_equal(_bits(dr.cpu().numpy().view(M.PROJ_DTYPE)[sel]), _bits(om.find_projections(pts[sel])))
 
+    say("cancellation with parts in flight, then the same calls again")
+    with pytest.raises(M.MeshLibAmdError) as ei:
+        fwn.calcFromGridWithDistancesByParts(lambda data, d, z0: False, dims, xf, opts, 1)
+    assert ei.value.code == M.MI_E_CANCELED
+    with pytest.raises(M.MeshLibAmdError) as ei:
+        fwn.calcFromVector(pts, 2.0, cb=lambda p: p < 0.7)
+    assert ei.value.code == M.MI_E_CANCELED
+    with pytest.raises(M.MeshLibAmdError) as ei:
+        fwn.calcFromGrid(dims, xf, 2.0, cb=lambda p: p < 0.65)
+    assert ei.value.code == M.MI_E_CANCELED
+    parts.clear()
+    fwn.calcFromGridWithDistancesByParts(lambda data, d, z0: parts.append((z0, d, data)), dims, xf, opts, 1)
+    assert [p[0] for p in parts] == [0, parts[0][1][2] - 1]
+    for z0, d, data in parts:
+        assert np.array_equal(_bits(data), _bits(full[z0 * layer:(z0 + d[2]) * layer]))
+    assert np.array_equal(_bits(fwn.calcFromVector(pts, 2.0)), _bits(dw.cpu().numpy()))
+
 
 def test_device_entry_points_match_host(k5):
     import torch
