@@ -224,6 +224,14 @@ def test_chunked_and_by_parts(k5):
             assert covered.all()
             if overlap:
                 assert all(parts[i + 1][0] == parts[i][0] + parts[i][1][2] - overlap for i in range(len(parts) - 1))
+        say("winding numbers by parts, overlap 1")
+        wfull = fwn.calcFromGrid(dims, xf, 2.0)
+        parts = []
+        fwn.calcFromGridByParts(lambda data, d, z0: parts.append((z0, d, data)), dims, xf, 2.0, 1)
+        assert len(parts) > 2
+        for z0, d, data in parts:
+            assert np.array_equal(_bits(data), _bits(wfull[z0 * layer:(z0 + d[2]) * layer]))
+        assert parts[-1][0] + parts[-1][1][2] == 40
         # canceling through the progress callback
         with pytest.raises(M.MeshLibAmdError) as ei:
             fwn.calcFromGrid(dims, xf, 2.0, cb=lambda p: p < 0.7)
@@ -260,6 +268,15 @@ def test_part_pipelines_at_full_budget(k5):
     for z0, d, data in parts:
         assert np.array_equal(_bits(data), _bits(full[z0 * layer:(z0 + d[2]) * layer]))
     assert parts[-1][0] + parts[-1][1][2] == 300
+    say("winding numbers by parts, overlap 3")
+    fwn.fwn_grid_device(dims, xf, 2.0, dev.data_ptr())
+    wfull = dev.cpu().numpy()
+    wparts = []
+    fwn.calcFromGridByParts(lambda data, d, z0: wparts.append((z0, d, data)), dims, xf, 2.0, 3)
+    assert len(wparts) == 2 and wparts[1][0] == wparts[0][1][2] - 3
+    for z0, d, data in wparts:
+        assert np.array_equal(_bits(data), _bits(wfull[z0 * layer:(z0 + d[2]) * layer]))
+    assert wparts[-1][0] + wparts[-1][1][2] == 300
 
     n = (2 << 21) + 4321                                      # two full parts and a ragged one
     pts = synth.projection_queries(v, f, n, 12)
