@@ -382,3 +382,50 @@ def test_config2_scale_subsample():
     assert np.array_equal(np.abs(gs[ids]), np.abs(os_))
     far = np.abs(os_) > 1e-3 * float(vox.min())
     assert np.array_equal(np.sign(gs[ids][far]), np.sign(os_[far]))
+
+
+def test_argument_and_budget_errors(k5):
+    """Invalid arguments and budgets come back as status codes with a message (no launch)."""
+    import ctypes
+    import torch
+    v, f, gm, om = k5
+    fwn = M.FastWindingNumber(gm)
+    opts = M.DistanceToMeshOptions()
+    dims, xf, vox = synth.grid_for_mesh(v, (32, 24, 40))
+
+    def code(fn):
+        with pytest.raises(M.MeshLibAmdError) as ei:
+            fn()
+        assert str(ei.value)
+        return ei.value.code
+    assert code(lambda: fwn.calcFromGridWithDistancesByParts(lambda *a: None, dims, xf, opts, -1)) == M.MI_E_INVALID
+    d = torch.empty(32 * 24 * 40, dtype=torch.float32, device="cuda")
+    assert code(lambda: fwn.sdf_grid_device(dims, xf, opts, d.data_ptr(), zBegin=5, zEnd=41)) == M.MI_E_INVALID
+    assert code(lambda: fwn.sdf_grid_device(dims, xf, opts, 0)) == M.MI_E_INVALID
+    caps = torch.ones(32 * 24 * 40, dtype=torch.float32, device="cuda")
+    band = M.DistanceToMeshOptions(minDistSq=0.001)
+    assert code(lambda: fwn.sdf_grid_capped_device(dims, xf, band, caps.data_ptr(), d.data_ptr())) == M.MI_E_INVALID
+    L = M.lib()
+    xf12 = M._xf(xf)
+    out = np.empty(32 * 24 * 40, np.float32)
+    o = opts._c()
+    neg = np.array([32, -1, 40], np.int32)
+    no_cb = M._progress(None)[0]
+    assert L.mi_fwn_calc_from_grid_with_distances(gm.handle, M._p(neg), M._p(xf12), ctypes.byref(o), M._p(out),
+                                                  no_cb, None) == M.MI_E_INVALID
+    good = np.array([32, 24, 40], np.int32)
+    assert L.mi_fwn_calc_from_grid(gm.handle, M._p(good), M._p(xf12), 2.0, None, no_cb, None) == M.MI_E_INVALID
+    try:
+        M.set_memory_limit(32 * 24 * 4 * 2 * 3)       # 3 layers per by-parts buffer of winding numbers
+        assert code(lambda: fwn.calcFromGridByParts(lambda *a: None, dims, xf, 2.0, 3)) == M.MI_E_LIMIT
+        parts = []
+        fwn.calcFromGridByParts(lambda data, dd, z0: parts.append(z0), dims, xf, 2.0, 2)
+        assert parts[:3] == [0, 1, 2]                 # step = 3 layers - 2 overlapping
+        M.set_memory_limit(100)                       # not one z-layer
+        assert code(lambda: fwn.calcFromGrid(dims, xf, 2.0)) == M.MI_E_LIMIT
+        assert code(lambda: M.PointsToMeshProjector().findProjections(v[:4])) == M.MI_E_INVALID
+    finally:
+        M.set_memory_limit(0)
+    # the mesh still works after the errors
+    g = fwn.calcFromGridWithDistances(dims, xf, opts)
+    assert np.array_equal(np.abs(g).view(np.uint32), np.abs(om.sdf_grid(dims, xf)).view(np.uint32))
