@@ -718,7 +718,7 @@ def main():
         return summarize_pmc(*args.summarize_pmc)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: this build has no multi-GPU path (asked for --gpus "
-              f"{args.gpus}); the path shards by z-slabs / query ranges with no exchange, see DESIGN.md",
+              f"{args.gpus}); the path would shard by z-slabs / query ranges with no exchange between GPUs",
               file=sys.stderr)
         sys.exit(2)
     if args.profile_run:

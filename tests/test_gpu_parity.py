@@ -1,6 +1,6 @@
 """GPU parity of the HIP path against the CPU oracle, through the C-ABI.
 
-Bar (BASELINE north star): projection records bit-identical (face ids, points, barycentrics,
+Bar: projection records bit-identical (face ids, points, barycentrics,
 distSq -- ties included, since the per-lane visit order is the reference's); signed-distance
 magnitudes bit-identical; winding numbers within 1e-4 absolute (they differ only by atan2f
 ulps); inside/outside sign identical on every voxel farther than 1e-3 voxel from the surface.
