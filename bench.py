@@ -269,11 +269,11 @@ def cpu_baseline(np, synth, v, f, dims, xf, pts, nvox):
             "cpu_model": cpu_model(), "affinity_cpus": aff, "cgroup_cpu_quota": quota,
             "method": "oracle (C restatement of the reference CPU path, -O2 -ffp-contract=off) with a "
                       "ParallelFor of dynamic contiguous chunks on all threads; 1 warm-up + median of 5",
-            "sample": f"config 2: {len(zs)} whole z-layers of the 256^3 grid (z = {zs[0]}, {zs[1]}, ..., "
+            "sample": f"k=8 256^3 step: {len(zs)} whole z-layers of the 256^3 grid (z = {zs[0]}, {zs[1]}, ..., "
                       f"{zs[-1]}: {ids.size} voxels) and 2 x 2^17 contiguous queries (one range of each "
                       f"half of the 2^22 set); step time = voxels / voxel rate + points / point rate",
             "voxels_per_s": vox_rate, "pts_per_s": pts_rate,
-            "config1_full": {"workload": "k=5 (20,480 tris, 8 bumps), 64^3 signed-distance grid",
+            "k5_64_full": {"workload": "k=5 (20,480 tris, 8 bumps), 64^3 signed-distance grid",
                              "voxels_per_s": 64 ** 3 / t1, "ms": t1 * 1e3},
             "one_thread": {"voxels_per_s": vox1, "pts_per_s": pts1,
                            "sample": f"{ids1.size} contiguous voxels + 16384 contiguous queries"}}
@@ -717,7 +717,7 @@ def main():
     if args.summarize_pmc:
         return summarize_pmc(*args.summarize_pmc)
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only: this build has no multi-GPU path (asked for --gpus "
+        print(f"bench.py measures one GPU only: the library has no multi-GPU path (asked for --gpus "
               f"{args.gpus}); the path would shard by z-slabs / query ranges with no exchange between GPUs",
               file=sys.stderr)
         sys.exit(2)

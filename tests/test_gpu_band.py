@@ -100,8 +100,8 @@ def test_projections_lo_bounded_device(restore_modes):
 
 
 @pytest.mark.parametrize("null_outside", [False, True])
-def test_sdf_band_config1(null_outside, restore_modes):
-    # config-1 size: k=5 mesh, 64^3 grid, the offset caller's band around a 3-voxel offset
+def test_sdf_band_k5_64(null_outside, restore_modes):
+    # the small CPU-runnable size: k=5 mesh, 64^3 grid, the offset caller's band around a 3-voxel offset
     v, f = synth.bumpy_icosphere(5, 8, 1)
     dims, xf, vox = synth.grid_for_mesh(v, (64, 64, 64))
     gm, om = M.Mesh(v, f), O.Mesh(v, f)

@@ -34,7 +34,7 @@ def _bits(a):
     return np.ascontiguousarray(a).view(np.uint8)
 
 
-def test_projection_bit_identical_config1(k5):
+def test_projection_bit_identical_k5(k5):
     v, f, gm, om = k5
     dims, xf, vox = synth.grid_for_mesh(v, (64, 64, 64))
     pts = np.concatenate([O.grid_points(dims, xf), synth.projection_queries(v, f, 65536, 11)])
@@ -78,7 +78,7 @@ def test_projection_limits_and_xf(k5):
     assert np.max(np.abs(np.sqrt(g["distSq"].astype(np.float64)) - np.sqrt(d64))) < 2e-6
 
 
-def test_sdf_config1(k5):
+def test_sdf_k5_64(k5):
     v, f, gm, om = k5
     dims, xf, vox = synth.grid_for_mesh(v, (64, 64, 64))
     say("sdf 64^3 k=5")
@@ -361,7 +361,7 @@ def test_degenerate_meshes():
     assert np.abs(g - om.fwn_vector(pts)).max() <= FWN_TOL
 
 
-def test_config2_scale_subsample():
+def test_k8_scale_subsample():
     # production mesh (k=8, 1.31M triangles): seeded subsamples against the oracle
     say("k=8 build")
     v, f = synth.bumpy_icosphere(8, 32, 2)
